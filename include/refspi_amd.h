@@ -1,0 +1,142 @@
+/* C-ABI of the RefSPI-MaxLP reconstruction hot path on MI355X (gfx950).
+ *
+ * Conventions
+ *  - Every entry point returns an int status: 0 = OK, < 0 = argument/state error,
+ *    > 0 = a hipError_t from the runtime.  rsp_last_error() returns the message of the
+ *    most recent failure on the calling thread.
+ *  - Device buffers are plain pointers in device memory owned by the caller; complex
+ *    arrays are interleaved (re, im) doubles, as numpy/CuPy 'c16'.  Host buffers are
+ *    marked "host".  Sizes are int64_t, scalars double.
+ *  - Calls that take an rsp_stream_t (a hipStream_t; NULL = default stream) are
+ *    asynchronous on that stream unless stated otherwise.
+ *  - Plans (rsp_emc, rsp_maxlp) own their scratch device memory; one plan per host
+ *    thread at a time (thread-compatible, not thread-safe).
+ */
+#ifndef REFSPI_AMD_H
+#define REFSPI_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* rsp_stream_t;
+typedef struct rsp_emc rsp_emc;
+typedef struct rsp_maxlp rsp_maxlp;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* rsp_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int rsp_version(void);
+
+/* ---------------------------------------------------------------------------------
+ * Single-kernel mirrors of the reference's CuPy RawKernels (FP64, device pointers).
+ * ------------------------------------------------------------------------------- */
+
+/* kernels.cu:6-42 slice_gen: bilinear in-plane rotation of a real size x size image,
+ * times scale plus bg, optional log (<1e-20 or out of bounds -> -1000).  cos/sin of
+ * `angle` are taken on the host with the C library. */
+int rsp_slice_gen(const double* model, double angle, double scale, int64_t size,
+                  const double* bg, int64_t log_flag, double* view, rsp_stream_t stream);
+
+/* kernels.cu:44-82 slice_gen_holo: |model + rel*S(s)*e^{i phase}|^2 * scale + bg. */
+int rsp_slice_gen_holo(const double* model_c, double shiftx, double shifty,
+                       double diameter, double rel_scale, double scale, int64_t size,
+                       const double* bg, int64_t log_flag, double* view,
+                       rsp_stream_t stream);
+
+/* kernels.cu:116-137 calc_prob_all: per-frame sparse Poisson log-likelihood, FP64,
+ * sequential per frame in the reference order (init*scale, ones, multi). */
+int rsp_calc_prob_all(const double* lview, const int32_t* mask, int64_t ndata,
+                      const int32_t* ones, const int32_t* multi, const int64_t* o_acc,
+                      const int64_t* m_acc, const int32_t* p_o, const int32_t* p_m,
+                      const int32_t* c_m, double init, const double* scales,
+                      double* prob_r, rsp_stream_t stream);
+
+/* kernels.cu:172-183 get_f_dt: f_dt[d*npix + t] = fobj_t[t] + fref_d[d] (complex). */
+int rsp_get_f_dt(const double* fobj_t, const double* fref_d, int64_t ndata, int64_t npix,
+                 double* f_dt, rsp_stream_t stream);
+
+/* ---------------------------------------------------------------------------------
+ * EMC E-step plan: replaces emc.py:159-216 (_calculate_prob + _get_rmax) for one GPU.
+ * Hypotheses are h = s*num_rot + r (s = state index of the raveled 'ij' meshgrid of
+ * emc.py:71-77, r = rotation j*pi/num_rot); the result is the first-occurrence argmax
+ * over h of the FP64 log-likelihood of kernels.cu:116-137.
+ * ------------------------------------------------------------------------------- */
+
+/* Create a plan on `device`.  rot_cos/rot_sin (host, num_rot) are cos/sin of
+ * j*pi/num_rot; state_sx/sy/dia (host, num_states) the raveled state grid. */
+int rsp_emc_create(int device, int64_t size, int64_t num_rot, const double* rot_cos,
+                   const double* rot_sin, const double* state_sx, const double* state_sy,
+                   const double* state_dia, int64_t num_states, rsp_emc** out);
+
+/* Load the sparse frames (host arrays, dset.py:23-60 layout; bg may be NULL = zeros).
+ * The plan copies what it needs; the caller may free the arrays afterwards.
+ * Synchronous. */
+int rsp_emc_set_frames(rsp_emc* plan, int64_t num_data, const int32_t* ones,
+                       const int32_t* multi, const int32_t* place_ones,
+                       const int32_t* place_multi, const int32_t* count_multi,
+                       const double* bg);
+
+/* Tuning / mode switches.  Keys:
+ *   1 = exact mode (1: every frame searched in FP64 over all hypotheses; default 0)
+ *   2 = hypotheses per table batch (default chosen from memory)
+ *   3 = LDS bank count assumed by the photon scheduler (32 or 64; set before frames)
+ *   4 = hypothesis segments per batch (default automatic)
+ *   5 = per-kernel timing with HIP events (1 on; read with rsp_emc_timing) */
+int rsp_emc_set_option(rsp_emc* plan, int key, int64_t value);
+
+/* Run the E-step for `model_c` (device, npix complex) with per-frame `scales` (device,
+ * N).  Writes rmax (device int32 N: global hypothesis index), maxll (device f64 N: the
+ * FP64 log-likelihood of that hypothesis), msums (device f64 num_states: emc.py:177) and
+ * *vscale (host, emc.py:181).  Synchronises `stream` before returning. */
+int rsp_emc_estep(rsp_emc* plan, const double* model_c, const double* scales,
+                  int32_t* rmax, double* maxll, double* msums, double* vscale,
+                  rsp_stream_t stream);
+
+/* Counters of the last E-step (host int64[8]): [0] frames, [1] hypotheses,
+ * [2] frames decided by the FP32-table pass, [3] frames resolved by top-2 FP64
+ * re-evaluation, [4] frames re-searched in full in FP64, [5] slots per lane (P),
+ * [6] waves per workgroup, [7] table parts. */
+int rsp_emc_stats(const rsp_emc* plan, int64_t* out);
+
+/* Timing of the last E-step with option 5 on (host double[4]): [0] total ms of the
+ * table kernels, [1] total ms of the gather kernels, [2] batches (= launches of each),
+ * [3] hypotheses per batch. */
+int rsp_emc_timing(const rsp_emc* plan, double* out);
+
+int rsp_emc_destroy(rsp_emc* plan);
+
+/* ---------------------------------------------------------------------------------
+ * MaxLP plan: replaces maxLP.py:304-340 (MaxLPhaser._run_phaser) for one GPU.
+ * ------------------------------------------------------------------------------- */
+
+/* Create a plan for a size x size model (maxLP.py:14-35 geometry, masks and pattern). */
+int rsp_maxlp_create(int device, int64_t size, rsp_maxlp** out);
+
+/* Load the sparse frames (host arrays, as rsp_emc_set_frames; maxLP.py:50-71).
+ * Synchronous. */
+int rsp_maxlp_set_frames(rsp_maxlp* plan, int64_t num_data, const int32_t* ones,
+                         const int32_t* multi, const int32_t* place_ones,
+                         const int32_t* place_multi, const int32_t* count_multi);
+
+/* Pattern-search update of model_c (device, npix complex, in place) given the per-frame
+ * shift (sx, sy), diameter and cos/sin of the negated angle (device f64 N each:
+ * rot_cos = cos(-ang), rot_sin = sin(-ang), maxLP.py:311).  Writes the ring rescale of
+ * maxLP.py:324-329 to *rescale_out (host, may be NULL).  Synchronises `stream`. */
+int rsp_maxlp_run(rsp_maxlp* plan, double* model_c, const double* sx, const double* sy,
+                  const double* dia, const double* rot_cos, const double* rot_sin,
+                  double* rescale_out, rsp_stream_t stream);
+
+/* Sizes of the plan (host int64[5]): frames, photon entries, model-space photon entries
+ * of the last run, ring pixels, mask pixels. */
+int rsp_maxlp_info(const rsp_maxlp* plan, int64_t* out);
+
+int rsp_maxlp_destroy(rsp_maxlp* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REFSPI_AMD_H */

@@ -1,0 +1,998 @@
+// EMC E-step plan for gfx950: the fused replacement of emc.py:159-216.
+//
+// Reference path (emc.py:171-199, kernels.cu:44-137): for every state s and rotation r
+// build the log of the rotated hologram and sum it over each frame's scored photons,
+// then take the first-occurrence argmax over h = s*R + r.
+//
+// Device pipeline (one GPU, no state x rotation array ever reaches HBM):
+//   1. k_holo_states   I_s = |O + S_s*ramp_s|^2 + bg for all states (FP64) + msums_s.
+//   2. k_vscale_init   vscale (emc.py:181) and init_s = -(msums_s*vscale).
+//   3. per batch of hypotheses:
+//      k_tables        FP32 log-view tables over the scored annulus (kernels.cu:6-42 on
+//                      I_s), plus the per-pixel maximum FP32 rounding error.
+//      k_gather<P,NW>  one workgroup holds 64*NW frames whose scored photons live in
+//                      VGPRs as LDS byte addresses, bank-scheduled on the host so that
+//                      each wave-wide ds_read hits distinct banks; per hypothesis the
+//                      table is copied to LDS and every lane gathers its photons,
+//                      accumulating in FP64 and keeping a running top-3.
+//      k_merge_top3    fold the per-segment top-3 into the running per-frame state.
+//   4. k_resolve       per frame: a rigorous bound E_d on the FP32-table error decides
+//                      whether the FP32 ranking is certain; otherwise the top-2 are
+//                      re-evaluated in FP64, or (rare) the frame is re-searched over all
+//                      hypotheses in FP64 (k_full_search).  maxll is always the FP64
+//                      log-likelihood of the chosen hypothesis, summed in the reference
+//                      order (ones then multi, kernels.cu:126-136).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "refspi_amd.h"
+#include "rsp_common.hpp"
+
+namespace rsp {
+
+constexpr int kNPMax = 4;        // table parts (LDS-sized pieces of the scored annulus)
+constexpr int kPartMax = 38400;  // floats per part: 150 KB of the 160 KB LDS
+
+struct Top3 {
+  double v1, v2, v3;
+  int h1, h2;
+};
+
+__device__ __forceinline__ bool better(double a, int ha, double b, int hb) {
+  return a > b || (a == b && ha < hb);
+}
+
+__device__ __forceinline__ void top3_insert(Top3& t, double v, int h) {
+  if (better(v, h, t.v1, t.h1)) {
+    t.v3 = t.v2;
+    t.v2 = t.v1;
+    t.h2 = t.h1;
+    t.v1 = v;
+    t.h1 = h;
+  } else if (better(v, h, t.v2, t.h2)) {
+    t.v3 = t.v2;
+    t.v2 = v;
+    t.h2 = h;
+  } else if (v > t.v3) {
+    t.v3 = v;
+  }
+}
+
+__device__ __forceinline__ Top3 top3_empty() {
+  Top3 t;
+  t.v1 = t.v2 = t.v3 = -INFINITY;
+  t.h1 = t.h2 = 0x7fffffff;
+  return t;
+}
+
+// --------------------------------------------------------------------------------
+// 1-2: holograms, msums, vscale
+// --------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_holo_states(
+    const double* __restrict__ model_c, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ dia,
+    const double* __restrict__ bg, const int32_t* __restrict__ probmask, int size,
+    double* __restrict__ I, double* __restrict__ msums) {
+  __shared__ double red[256];
+  const int s = blockIdx.x;
+  const int npix = size * size;
+  const double ssx = sx[s], ssy = sy[s], sdia = dia[s];
+  double* Is = I + (int64_t)s * npix;
+  double acc = 0.;
+  for (int t = threadIdx.x; t < npix; t += 256) {
+    const int x = t / size, y = t - x * size;
+    double v = holo_pixel(model_c, x, y, size, ssx, ssy, sdia, 1.);
+    v = v * 1.;
+    v = v + bg[t];
+    Is[t] = v;
+    if (probmask[t] < 1) acc += v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) msums[s] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_vscale_init(const double* __restrict__ msums, int S,
+                                                     double powder_sel_sum,
+                                                     double* __restrict__ init,
+                                                     double* __restrict__ vscale_out) {
+  __shared__ double red[256];
+  double acc = 0.;
+  for (int s = threadIdx.x; s < S; s += 256) acc += msums[s];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double vscale = powder_sel_sum / red[0] * (double)S;
+  if (threadIdx.x == 0) *vscale_out = vscale;
+  for (int s = threadIdx.x; s < S; s += 256) init[s] = -(msums[s] * vscale);
+}
+
+// --------------------------------------------------------------------------------
+// exact FP64 log-view value of detector pixel t for hypothesis (s, r)
+// (kernels.cu:6-42 with log_flag = 1, scale = 1, applied to I_s which already holds bg)
+// --------------------------------------------------------------------------------
+__device__ __forceinline__ double logview_exact(const double* __restrict__ Is,
+                                                const double* __restrict__ bg, int size,
+                                                int t, double ac, double as) {
+  const int x = t / size, y = t - x * size;
+  double tx, ty, v;
+  rot_source(x, y, size / 2, ac, as, tx, ty);
+  if (!bilinear(Is, size, tx, ty, v)) return -1000.;
+  v = v * 1.;
+  v = v + bg[t];
+  return log_view(v);
+}
+
+// --------------------------------------------------------------------------------
+// 3a: FP32 tables for hypotheses [hb, hb+nb)
+// tables[(h-hb)*TS + part*PSTRIDE + (i - part*kPartMax)] = fl32(logview)
+// --------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tables(
+    const double* __restrict__ I, const double* __restrict__ bg,
+    const int32_t* __restrict__ scored_t, int n_scored, int size, int R,
+    const double* __restrict__ rcos, const double* __restrict__ rsin, int hb, int nb,
+    int HC, int PSTRIDE, int TS, float* __restrict__ tables,
+    unsigned* __restrict__ errmax) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_scored) return;
+  const int t = scored_t[i];
+  const int npix = size * size;
+  const int part = i / kPartMax;
+  const int64_t col = (int64_t)part * PSTRIDE + (i - part * kPartMax);
+  const int h0 = hb + blockIdx.y * HC;
+  const int h1 = min(hb + nb, h0 + HC);
+  float emax = 0.f;
+  for (int h = h0; h < h1; ++h) {
+    const int s = h / R, r = h - s * R;
+    const double lv = logview_exact(I + (int64_t)s * npix, bg, size, t, rcos[r], rsin[r]);
+    const float f = (float)lv;
+    tables[(int64_t)(h - hb) * TS + col] = f;
+    emax = fmaxf(emax, (float)fabs((double)f - lv));
+  }
+  // max of non-negative floats == max of their bit patterns: order independent
+  atomicMax(errmax + i, __float_as_uint(emax));
+}
+
+// Compile-time walk over the slot registers in blocks of 8 so that every a[k] is a
+// statically named VGPR (a runtime-indexed loop would leave the array in scratch).
+template <int K0, int P>
+__device__ __forceinline__ void gather_blocks(const char* lds, const uint32_t (&a)[P], int kbeg,
+                                              int kend, double& acc0, double& acc1) {
+  if constexpr (K0 < P) {
+    if (K0 >= kend) return;
+    if (K0 >= kbeg) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K0 + u]);
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        acc0 += (double)v[u];
+        acc1 += (double)v[u + 1];
+      }
+    }
+    gather_blocks<K0 + 8, P>(lds, a, kbeg, kend, acc0, acc1);
+  }
+}
+
+template <int K0, int P>
+__device__ __forceinline__ void load_slots(uint32_t (&a)[P], const uint32_t* __restrict__ src) {
+  if constexpr (K0 < P) {
+    a[K0] = src[(int64_t)K0 * 64];
+    load_slots<K0 + 1, P>(a, src);
+  }
+}
+
+// --------------------------------------------------------------------------------
+// 3b: the gather.  Frames are processed in "sorted order" j (by scored-photon count);
+// wave gw holds frames j = 64*gw + lane.  slots[(gw*P + k)*64 + lane] is the LDS byte
+// address (within the table part resident in LDS) of the frame's k-th photon entry;
+// wave_meta[gw*kNPMax + p] is the end slot of part p (a multiple of 8).
+// --------------------------------------------------------------------------------
+template <int P, int NW>
+__global__ __launch_bounds__(64 * NW) void k_gather(
+    const float* __restrict__ tables, int TS, int PSTRIDE, int NP, int hb, int nb,
+    int nseg, int n_tiles, const uint32_t* __restrict__ slots,
+    const int32_t* __restrict__ wave_meta, const double* __restrict__ scale_sorted,
+    const double* __restrict__ init, int R, Top3* __restrict__ partials, int Npad) {
+  extern __shared__ float4 lds4[];
+  const char* lds = reinterpret_cast<const char*>(lds4);
+  const int b = blockIdx.x;
+  int seg, tile;
+  if ((nseg & 7) == 0) {  // keep one segment's workgroups on one XCD (round-robin dispatch)
+    const int q = b >> 3;
+    seg = (b & 7) + 8 * (q / n_tiles);
+    tile = q % n_tiles;
+  } else {
+    seg = b / n_tiles;
+    tile = b % n_tiles;
+  }
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gw = tile * NW + wid;
+  const int j = gw * 64 + lane;
+  const int h0 = hb + (int)((int64_t)seg * nb / nseg);
+  const int h1 = hb + (int)((int64_t)(seg + 1) * nb / nseg);
+
+  uint32_t a[P];
+  load_slots<0, P>(a, slots + (int64_t)gw * P * 64 + lane);
+  int pend[kNPMax];
+#pragma unroll
+  for (int p = 0; p < kNPMax; ++p) pend[p] = wave_meta[gw * kNPMax + p];
+  const double scale = scale_sorted[j];
+  Top3 top = top3_empty();
+  const int n4 = PSTRIDE >> 2;
+
+  for (int h = h0; h < h1; ++h) {
+    const double base = init[h / R] * scale;
+    double acc0 = base, acc1 = 0.;
+    int kbeg = 0;
+    for (int p = 0; p < NP; ++p) {
+      const float4* src =
+          reinterpret_cast<const float4*>(tables + (int64_t)(h - hb) * TS + (int64_t)p * PSTRIDE);
+      __syncthreads();
+      for (int i = threadIdx.x; i < n4; i += 64 * NW) lds4[i] = src[i];
+      __syncthreads();
+      const int kend = pend[p];
+      gather_blocks<0, P>(lds, a, kbeg, kend, acc0, acc1);
+      kbeg = kend;
+    }
+    top3_insert(top, acc0 + acc1, h);
+  }
+  partials[(int64_t)seg * Npad + j] = top;
+}
+
+__global__ __launch_bounds__(256) void k_merge_top3(const Top3* __restrict__ partials, int nseg,
+                                                    int Npad, Top3* __restrict__ running,
+                                                    int first) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= Npad) return;
+  Top3 t = first ? top3_empty() : running[j];
+  for (int g = 0; g < nseg; ++g) {
+    const Top3 p = partials[(int64_t)g * Npad + j];
+    top3_insert(t, p.v1, p.h1);
+    top3_insert(t, p.v2, p.h2);
+    if (p.v3 > t.v3) t.v3 = p.v3;
+  }
+  running[j] = t;
+}
+
+// --------------------------------------------------------------------------------
+// 4: exact resolution.  Per frame (frame id d), the scored photons in reference order:
+// ex_off[d]..ex_off[d]+ex_n1[d] are 'ones', the rest up to ex_off[d+1] 'multi'.
+// --------------------------------------------------------------------------------
+struct ExactArgs {
+  const double* I;
+  const double* bg;
+  const double* rcos;
+  const double* rsin;
+  const double* init;
+  const double* scales;
+  const int64_t* ex_off;
+  const int32_t* ex_n1;
+  const int32_t* ex_pix;
+  const int32_t* ex_cnt;
+  int size, R;
+};
+
+__device__ double exact_ll(const ExactArgs& A, int d, int h) {
+  const int s = h / A.R, r = h - s * (A.R);
+  const double* Is = A.I + (int64_t)s * A.size * A.size;
+  const double ac = A.rcos[r], as = A.rsin[r];
+  double acc = A.init[s] * A.scales[d];
+  const int64_t e0 = A.ex_off[d], e1 = A.ex_off[d + 1], em = e0 + A.ex_n1[d];
+  for (int64_t e = e0; e < em; ++e) acc += logview_exact(Is, A.bg, A.size, A.ex_pix[e], ac, as);
+  for (int64_t e = em; e < e1; ++e)
+    acc += logview_exact(Is, A.bg, A.size, A.ex_pix[e], ac, as) * A.ex_cnt[e];
+  return acc;
+}
+
+// Per sorted frame j: decide, re-evaluate the top-2 in FP64 if needed, or queue a full
+// FP64 search.  errmax holds per scored pixel the max |fl32(lv) - lv| over hypotheses.
+__global__ __launch_bounds__(256) void k_resolve(ExactArgs A, const Top3* __restrict__ running,
+                                                 const int32_t* __restrict__ perm, int Npad,
+                                                 const int32_t* __restrict__ compact_of,
+                                                 const unsigned* __restrict__ errmax,
+                                                 int exact_mode, int32_t* __restrict__ rmax,
+                                                 double* __restrict__ maxll,
+                                                 int32_t* __restrict__ full_list,
+                                                 unsigned* __restrict__ counters) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= Npad) return;
+  const int d = perm[j];
+  if (d < 0) return;
+  if (exact_mode) {
+    full_list[atomicAdd(counters + 2, 1u)] = j;
+    return;
+  }
+  const Top3 t = running[j];
+  // bound on |LL_fp32tables - LL_fp64| for any hypothesis of this frame
+  double E = 0.;
+  for (int64_t e = A.ex_off[d]; e < A.ex_off[d + 1]; ++e) {
+    const int c = (e < A.ex_off[d] + A.ex_n1[d]) ? 1 : A.ex_cnt[e];
+    E += (double)c * (double)__uint_as_float(errmax[compact_of[A.ex_pix[e]]]);
+  }
+  // plus the FP64 summation error of both paths: n*2^-53*sum|terms| with |log-view| <= 100
+  // (values of -1000 mark out-of-range samples and never win); 1000x safety factor
+  const double n = (double)(A.ex_off[d + 1] - A.ex_off[d]) + 1.;
+  E = 2. * E + 1.e-13 * n * (fabs(t.v1) + 100. * n + 1.);
+  int h = t.h1;
+  double ll;
+  if (E == 0. || t.v1 - t.v2 > 2. * E) {
+    ll = exact_ll(A, d, h);
+    atomicAdd(counters + 0, 1u);
+  } else if (t.v1 - t.v3 > 2. * E) {
+    const double l1 = exact_ll(A, d, t.h1), l2 = exact_ll(A, d, t.h2);
+    if (better(l2, t.h2, l1, t.h1)) {
+      h = t.h2;
+      ll = l2;
+    } else {
+      ll = l1;
+    }
+    atomicAdd(counters + 1, 1u);
+  } else {
+    full_list[atomicAdd(counters + 2, 1u)] = j;
+    return;
+  }
+  rmax[d] = h;
+  maxll[d] = ll;
+}
+
+// Full FP64 search for the queued frames: block (bx, f) covers hypotheses
+// [bx*256, bx*256+256) of frame full_list[f]; per-block first-occurrence argmax.
+__global__ __launch_bounds__(256) void k_full_search(ExactArgs A, const int32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ full_list,
+                                                     int H, double* __restrict__ bval,
+                                                     int32_t* __restrict__ bidx, int nbx) {
+  __shared__ double sv[256];
+  __shared__ int si[256];
+  const int f = blockIdx.y;
+  const int d = perm[full_list[f]];
+  const int h = blockIdx.x * 256 + threadIdx.x;
+  double v = -INFINITY;
+  int hi = 0x7fffffff;
+  if (h < H) {
+    v = exact_ll(A, d, h);
+    hi = h;
+  }
+  sv[threadIdx.x] = v;
+  si[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      if (better(sv[threadIdx.x + w], si[threadIdx.x + w], sv[threadIdx.x], si[threadIdx.x])) {
+        sv[threadIdx.x] = sv[threadIdx.x + w];
+        si[threadIdx.x] = si[threadIdx.x + w];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bval[(int64_t)f * nbx + blockIdx.x] = sv[0];
+    bidx[(int64_t)f * nbx + blockIdx.x] = si[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_full_reduce(const int32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ full_list,
+                                                     int nfull, const double* __restrict__ bval,
+                                                     const int32_t* __restrict__ bidx, int nbx,
+                                                     int32_t* __restrict__ rmax,
+                                                     double* __restrict__ maxll) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
+  if (f >= nfull) return;
+  double v = -INFINITY;
+  int h = 0x7fffffff;
+  for (int b = 0; b < nbx; ++b) {
+    const double bv = bval[(int64_t)f * nbx + b];
+    const int bi = bidx[(int64_t)f * nbx + b];
+    if (better(bv, bi, v, h)) {
+      v = bv;
+      h = bi;
+    }
+  }
+  const int d = perm[full_list[f]];
+  rmax[d] = h;
+  maxll[d] = v;
+}
+
+__global__ void k_scatter_scale(const double* __restrict__ scales, const int32_t* __restrict__ perm,
+                                int Npad, double* __restrict__ scale_sorted) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= Npad) return;
+  const int d = perm[j];
+  scale_sorted[j] = d < 0 ? 0. : scales[d];
+}
+
+}  // namespace rsp
+
+// ====================================================================================
+// host side
+// ====================================================================================
+using namespace rsp;
+
+struct rsp_emc {
+  int device = 0;
+  int size = 0, npix = 0, R = 0, S = 0;
+  int64_t H = 0;
+  std::vector<double> rcos, rsin, ssx, ssy, sdia;
+  std::vector<int32_t> probmask, scored_t, compact_of;
+  int n_scored = 0, NP = 1, PSTRIDE = 0, TS = 0, NB = 32;
+  // device geometry
+  double *d_rcos = nullptr, *d_rsin = nullptr, *d_sx = nullptr, *d_sy = nullptr,
+         *d_dia = nullptr, *d_bg = nullptr, *d_I = nullptr, *d_init = nullptr,
+         *d_vscale = nullptr;
+  int32_t *d_probmask = nullptr, *d_scored_t = nullptr, *d_compact_of = nullptr;
+  float* d_tables = nullptr;
+  unsigned* d_errmax = nullptr;
+  int64_t batch = 0;
+  // frames
+  int64_t N = 0, Npad = 0;
+  int P = 0, NW = 0, n_waves = 0, n_tiles = 0;
+  double powder_sel_sum = 0.;
+  uint32_t* d_slots = nullptr;
+  int32_t *d_meta = nullptr, *d_perm = nullptr;
+  double* d_scale_sorted = nullptr;
+  Top3 *d_partials = nullptr, *d_running = nullptr;
+  int64_t partial_cap = 0;
+  int64_t* d_ex_off = nullptr;
+  int32_t *d_ex_n1 = nullptr, *d_ex_pix = nullptr, *d_ex_cnt = nullptr;
+  int32_t* d_full_list = nullptr;
+  unsigned* d_counters = nullptr;
+  double* d_bval = nullptr;
+  int32_t* d_bidx = nullptr;
+  int64_t bcap = 0;
+  // options / stats
+  int exact_mode = 0;
+  int64_t opt_batch = 0, opt_nseg = 0;
+  int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // optional per-kernel timing with HIP events on the plan's stream
+  int timing = 0;
+  std::vector<hipEvent_t> ev;  // 4 per batch: tables begin/end, gather begin/end
+  double t_tables_ms = 0., t_gather_ms = 0.;
+  int64_t n_batches = 0, gather_bytes_alg = 0;
+};
+
+namespace {
+
+template <typename T>
+int dev_alloc(T** p, int64_t n) {
+  if (*p) {
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  return kOk;
+}
+
+template <typename T>
+int dev_upload(T** p, const std::vector<T>& v) {
+  int rc = dev_alloc(p, (int64_t)v.size());
+  if (rc) return rc;
+  if (!v.empty()) RSP_HIP(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+void free_all(rsp_emc* e) {
+  void* ptrs[] = {e->d_rcos, e->d_rsin, e->d_sx, e->d_sy, e->d_dia, e->d_bg, e->d_I,
+                  e->d_init, e->d_vscale, e->d_probmask, e->d_scored_t, e->d_compact_of,
+                  e->d_tables, e->d_errmax, e->d_slots, e->d_meta, e->d_perm,
+                  e->d_scale_sorted, e->d_partials, e->d_running, e->d_ex_off, e->d_ex_n1,
+                  e->d_ex_pix, e->d_ex_cnt, e->d_full_list, e->d_counters, e->d_bval,
+                  e->d_bidx};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  for (hipEvent_t ev : e->ev) hipEventDestroy(ev);
+  e->ev.clear();
+}
+
+// Greedy bank-aware slot assignment for one group of lanes of one wave and one table
+// part.  entries[l] lists LDS entry indices (within the part) for lane l; the result
+// places every entry of lane l in a distinct slot so that, within the group, no two
+// lanes read the same bank in the same slot.  Returns the number of slots used.
+int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
+                   std::vector<std::vector<int>>& slot_of) {
+  const int G = (int)entries.size();
+  std::vector<uint64_t> bank_used;  // per slot, bit per bank
+  std::vector<std::vector<uint64_t>> lane_used(G);
+  slot_of.assign(G, {});
+  int max_len = 0;
+  for (int l = 0; l < G; ++l) {
+    slot_of[l].assign(entries[l].size(), -1);
+    max_len = std::max(max_len, (int)entries[l].size());
+  }
+  int nslots = 0;
+  auto lane_bit = [&](int l, int k) -> bool {
+    const auto& w = lane_used[l];
+    return (size_t)(k >> 6) < w.size() && ((w[k >> 6] >> (k & 63)) & 1ull);
+  };
+  for (int r = 0; r < max_len; ++r) {
+    for (int l = 0; l < G; ++l) {
+      if (r >= (int)entries[l].size()) continue;
+      const int bank = entries[l][r] % nbanks;
+      int k = 0;
+      for (;; ++k) {
+        if ((size_t)k >= bank_used.size()) bank_used.resize(k + 1, 0ull);
+        if (!((bank_used[k] >> bank) & 1ull) && !lane_bit(l, k)) break;
+      }
+      bank_used[k] |= 1ull << bank;
+      auto& w = lane_used[l];
+      if ((size_t)(k >> 6) >= w.size()) w.resize((k >> 6) + 1, 0ull);
+      w[k >> 6] |= 1ull << (k & 63);
+      slot_of[l][r] = k;
+      nslots = std::max(nslots, k + 1);
+    }
+  }
+  return nslots;
+}
+
+int choose_P(int need) {
+  static const int Ps[] = {8, 16, 32, 48, 64, 96, 128, 160, 192};
+  for (int p : Ps)
+    if (need <= p) return p;
+  return -1;
+}
+
+// waves per workgroup so that the P slot registers (+~38 working VGPRs) fit the
+// 512-entry VGPR file of a SIMD at NW/4 waves per SIMD
+int choose_NW(int P) {
+  if (P <= 64) return 16;
+  if (P <= 128) return 12;
+  return 8;
+}
+
+using GatherFn = void (*)(const float*, int, int, int, int, int, int, int, const uint32_t*,
+                          const int32_t*, const double*, const double*, int, Top3*, int);
+
+template <int P, int NW>
+GatherFn gfn() {
+  return &k_gather<P, NW>;
+}
+
+GatherFn pick_gather(int P, int NW) {
+#define RSP_G(p, nw) \
+  if (P == p && NW == nw) return gfn<p, nw>();
+  RSP_G(8, 16) RSP_G(16, 16) RSP_G(32, 16) RSP_G(48, 16) RSP_G(64, 16) RSP_G(96, 12)
+  RSP_G(128, 12) RSP_G(160, 8) RSP_G(192, 8)
+#undef RSP_G
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" int rsp_emc_create(int device, int64_t size, int64_t num_rot, const double* rot_cos,
+                              const double* rot_sin, const double* state_sx,
+                              const double* state_sy, const double* state_dia,
+                              int64_t num_states, rsp_emc** out) {
+  if (!out || !rot_cos || !rot_sin || !state_sx || !state_sy || !state_dia)
+    return fail(kErrNull, "rsp_emc_create: null pointer");
+  *out = nullptr;
+  if (size < 8 || size > 4096) return fail(kErrSize, "rsp_emc_create: size out of range");
+  if (num_rot < 1 || num_states < 1) return fail(kErrSize, "rsp_emc_create: empty grid");
+  if (num_rot * num_states > 0x7fffffffLL)
+    return fail(kErrSize, "rsp_emc_create: more than 2^31-1 hypotheses");
+  RSP_HIP(hipSetDevice(device));
+  rsp_emc* e = new rsp_emc();
+  e->device = device;
+  e->size = (int)size;
+  e->npix = (int)(size * size);
+  e->R = (int)num_rot;
+  e->S = (int)num_states;
+  e->H = num_rot * num_states;
+  e->rcos.assign(rot_cos, rot_cos + num_rot);
+  e->rsin.assign(rot_sin, rot_sin + num_rot);
+  e->ssx.assign(state_sx, state_sx + num_states);
+  e->ssy.assign(state_sy, state_sy + num_states);
+  e->sdia.assign(state_dia, state_dia + num_states);
+  // emc.py:89-98 probmask and the scored annulus (probmask == 0)
+  e->probmask.assign(e->npix, 0);
+  e->compact_of.assign(e->npix, -1);
+  const int c = e->size / 2;
+  for (int x = 0; x < e->size; ++x)
+    for (int y = 0; y < e->size; ++y) {
+      const double rad = std::sqrt((double)(x - c) * (x - c) + (double)(y - c) * (y - c));
+      int v = 0;
+      if (rad >= e->size / 2) v = 2;
+      if (rad < e->size / 8) v = 1;
+      if (rad < 4) v = 2;
+      e->probmask[x * e->size + y] = v;
+      if (v == 0) {
+        e->compact_of[x * e->size + y] = (int)e->scored_t.size();
+        e->scored_t.push_back(x * e->size + y);
+      }
+    }
+  e->n_scored = (int)e->scored_t.size();
+  e->NP = std::max(1, (int)ceil_div(e->n_scored, kPartMax));
+  int rc = 0;
+  if (e->NP > kNPMax) {
+    delete e;
+    return fail(kErrUnsupported, "rsp_emc_create: scored annulus too large for LDS parts");
+  }
+  const int part_len = std::min(e->n_scored, kPartMax);
+  e->PSTRIDE = (int)(ceil_div(part_len + 64, 4) * 4);  // + pad entries (zeros), 16 B rows
+  e->TS = e->PSTRIDE * e->NP;
+  if ((rc = dev_upload(&e->d_rcos, e->rcos)) || (rc = dev_upload(&e->d_rsin, e->rsin)) ||
+      (rc = dev_upload(&e->d_sx, e->ssx)) || (rc = dev_upload(&e->d_sy, e->ssy)) ||
+      (rc = dev_upload(&e->d_dia, e->sdia)) || (rc = dev_upload(&e->d_probmask, e->probmask)) ||
+      (rc = dev_upload(&e->d_scored_t, e->scored_t)) ||
+      (rc = dev_upload(&e->d_compact_of, e->compact_of)) ||
+      (rc = dev_alloc(&e->d_I, (int64_t)e->S * e->npix)) ||
+      (rc = dev_alloc(&e->d_init, e->S)) || (rc = dev_alloc(&e->d_vscale, 1)) ||
+      (rc = dev_alloc(&e->d_errmax, e->n_scored)) || (rc = dev_alloc(&e->d_counters, 4))) {
+    free_all(e);
+    delete e;
+    return rc;
+  }
+  *out = e;
+  return kOk;
+}
+
+extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
+  if (!e) return fail(kErrNull, "rsp_emc_set_option: null plan");
+  switch (key) {
+    case 1: e->exact_mode = value != 0; return kOk;
+    case 2:
+      if (value < 0) return fail(kErrSize, "batch < 0");
+      e->opt_batch = value;
+      return kOk;
+    case 3:
+      if (value != 32 && value != 64) return fail(kErrSize, "bank count must be 32 or 64");
+      e->NB = (int)value;
+      return kOk;
+    case 4:
+      if (value < 0) return fail(kErrSize, "segments < 0");
+      e->opt_nseg = value;
+      return kOk;
+    case 5: e->timing = value != 0; return kOk;
+    default: return fail(kErrSize, "rsp_emc_set_option: unknown key");
+  }
+}
+
+extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* ones,
+                                  const int32_t* multi, const int32_t* place_ones,
+                                  const int32_t* place_multi, const int32_t* count_multi,
+                                  const double* bg) {
+  if (!e) return fail(kErrNull, "rsp_emc_set_frames: null plan");
+  if (num_data < 1 || num_data > (1LL << 30)) return fail(kErrSize, "rsp_emc_set_frames: num_data");
+  if (!ones || !multi) return fail(kErrNull, "rsp_emc_set_frames: null counts");
+  RSP_HIP(hipSetDevice(e->device));
+  const int64_t N = num_data;
+  int64_t no = 0, nm = 0;
+  for (int64_t d = 0; d < N; ++d) {
+    if (ones[d] < 0 || multi[d] < 0) return fail(kErrSize, "negative photon count");
+    no += ones[d];
+    nm += multi[d];
+  }
+  if ((no && !place_ones) || (nm && (!place_multi || !count_multi)))
+    return fail(kErrNull, "rsp_emc_set_frames: null photon list");
+  for (int64_t i = 0; i < no; ++i)
+    if (place_ones[i] < 0 || place_ones[i] >= e->npix) return fail(kErrSize, "place_ones out of range");
+  for (int64_t i = 0; i < nm; ++i)
+    if (place_multi[i] < 0 || place_multi[i] >= e->npix)
+      return fail(kErrSize, "place_multi out of range");
+  e->N = N;
+  // background (dset.py:56-60) and powder (dset.py:74-83)
+  std::vector<double> hbg(e->npix, 0.);
+  if (bg) hbg.assign(bg, bg + e->npix);
+  std::vector<double> powder(e->npix, 0.);
+  for (int64_t i = 0; i < no; ++i) powder[place_ones[i]] += 1.;
+  for (int64_t i = 0; i < nm; ++i) powder[place_multi[i]] += count_multi[i];
+  double ps = 0.;
+  for (int t = 0; t < e->npix; ++t)
+    if (e->probmask[t] < 1) ps += powder[t] / (double)N;
+  e->powder_sel_sum = ps;
+  // exact-path CSR (reference order: scored ones, then scored multi)
+  std::vector<int64_t> ex_off(N + 1, 0);
+  std::vector<int32_t> ex_n1(N), ex_pix, ex_cnt;
+  std::vector<int> nentries(N, 0);  // entries with multiplicity for the FP32 path
+  {
+    int64_t po = 0, pm = 0;
+    for (int64_t d = 0; d < N; ++d) {
+      int n1 = 0, ne = 0;
+      for (int k = 0; k < ones[d]; ++k) {
+        const int t = place_ones[po + k];
+        if (e->probmask[t] < 1) {
+          ex_pix.push_back(t);
+          ex_cnt.push_back(1);
+          ++n1;
+          ++ne;
+        }
+      }
+      for (int k = 0; k < multi[d]; ++k) {
+        const int t = place_multi[pm + k];
+        if (e->probmask[t] < 1) {
+          ex_pix.push_back(t);
+          ex_cnt.push_back(count_multi[pm + k]);
+          ne += count_multi[pm + k];
+        }
+      }
+      po += ones[d];
+      pm += multi[d];
+      ex_n1[d] = n1;
+      ex_off[d + 1] = (int64_t)ex_pix.size();
+      nentries[d] = ne;
+    }
+  }
+  // sort frames by entry count (descending) into waves of 64
+  std::vector<int32_t> order(N);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return nentries[a] > nentries[b]; });
+  const int NB = e->NB;
+  const int G = NB;  // lanes per conflict group: 32 banks -> half waves, 64 -> full waves
+  const int64_t nw0 = ceil_div(N, 64);
+  // per wave, per part: slot assignments
+  std::vector<std::vector<std::vector<int>>> wave_slots(nw0);  // [w][lane] -> entries by slot
+  std::vector<std::vector<int>> wave_pend(nw0, std::vector<int>(kNPMax, 0));
+  int need = 0;
+  for (int64_t w = 0; w < nw0; ++w) {
+    std::vector<std::vector<int>> lane_slots(64);  // slot -> LDS entry (within part), -1 pad
+    int kbase = 0;
+    for (int p = 0; p < e->NP; ++p) {
+      int part_slots = 0;
+      std::vector<std::vector<int>> placed(64);  // per lane: slot -> entry
+      for (int g0 = 0; g0 < 64; g0 += G) {
+        std::vector<std::vector<int>> ent(G);
+        for (int l = 0; l < G; ++l) {
+          const int64_t jj = w * 64 + g0 + l;
+          if (jj >= N) continue;
+          const int d = order[jj];
+          for (int64_t q = ex_off[d]; q < ex_off[d + 1]; ++q) {
+            const int ci = e->compact_of[ex_pix[q]];
+            if (ci / kPartMax != p) continue;
+            const int local = ci - p * kPartMax;
+            const int c = (q < ex_off[d] + ex_n1[d]) ? 1 : ex_cnt[q];
+            for (int r = 0; r < c; ++r) ent[l].push_back(local);
+          }
+        }
+        std::vector<std::vector<int>> slot_of;
+        const int ns = schedule_group(ent, NB, slot_of);
+        part_slots = std::max(part_slots, ns);
+        for (int l = 0; l < G; ++l) {
+          auto& pl = placed[g0 + l];
+          pl.assign(ns, -1);
+          for (size_t r = 0; r < ent[l].size(); ++r) pl[slot_of[l][r]] = ent[l][r];
+        }
+      }
+      part_slots = (int)ceil_div(part_slots, 8) * 8;
+      const int part_len = std::min(kPartMax, e->n_scored - p * kPartMax);
+      for (int g0 = 0; g0 < 64; g0 += G) {
+        for (int k = 0; k < part_slots; ++k) {
+          uint64_t used = 0;
+          for (int l = 0; l < G; ++l) {
+            auto& pl = placed[g0 + l];
+            if (k < (int)pl.size() && pl[k] >= 0) used |= 1ull << (pl[k] % NB);
+          }
+          for (int l = 0; l < G; ++l) {
+            auto& pl = placed[g0 + l];
+            if ((int)pl.size() <= k) pl.resize(k + 1, -1);
+            if (pl[k] >= 0) continue;
+            int bank = 0;
+            while (bank < NB && ((used >> bank) & 1ull)) ++bank;
+            if (bank == NB) bank = 0;
+            used |= 1ull << bank;
+            pl[k] = part_len + ((bank - part_len) % NB + NB) % NB;  // a zero pad entry
+          }
+        }
+      }
+      for (int l = 0; l < 64; ++l)
+        for (int k = 0; k < part_slots; ++k) lane_slots[l].push_back(placed[l][k]);
+      kbase += part_slots;
+      wave_pend[w][p] = kbase;
+    }
+    for (int p = e->NP; p < kNPMax; ++p) wave_pend[w][p] = kbase;
+    wave_slots[w] = std::move(lane_slots);
+    need = std::max(need, kbase);
+  }
+  const int P = choose_P(std::max(need, 8));
+  if (P < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs more than 192 slots");
+  const int NW = choose_NW(P);
+  const int64_t n_tiles = ceil_div(nw0, NW);
+  const int64_t n_waves = n_tiles * NW;
+  const int64_t Npad = n_waves * 64;
+  std::vector<uint32_t> slots((size_t)(n_waves * P * 64));
+  std::vector<int32_t> meta((size_t)(n_waves * kNPMax));
+  std::vector<int32_t> perm(Npad, -1);
+  for (int64_t w = 0; w < n_waves; ++w) {
+    for (int p = 0; p < kNPMax; ++p) meta[w * kNPMax + p] = w < nw0 ? wave_pend[w][p] : 0;
+    for (int l = 0; l < 64; ++l) {
+      const int64_t jj = w * 64 + l;
+      if (jj < N) perm[jj] = order[jj];
+      for (int k = 0; k < P; ++k) {
+        int entry;
+        if (w < nw0 && k < (int)wave_slots[w][l].size())
+          entry = wave_slots[w][l][k];
+        else
+          entry = std::min(kPartMax, e->n_scored) + (l % 64);  // pad (never read if k >= end)
+        slots[((size_t)w * P + k) * 64 + l] = (uint32_t)entry * 4u;
+      }
+    }
+  }
+  int rc = 0;
+  std::vector<double> hb(hbg);
+  if ((rc = dev_upload(&e->d_bg, hb)) || (rc = dev_upload(&e->d_slots, slots)) ||
+      (rc = dev_upload(&e->d_meta, meta)) || (rc = dev_upload(&e->d_perm, perm)) ||
+      (rc = dev_upload(&e->d_ex_off, ex_off)) || (rc = dev_upload(&e->d_ex_n1, ex_n1)) ||
+      (rc = dev_upload(&e->d_ex_pix, ex_pix)) || (rc = dev_upload(&e->d_ex_cnt, ex_cnt)) ||
+      (rc = dev_alloc(&e->d_scale_sorted, Npad)) || (rc = dev_alloc(&e->d_running, Npad)) ||
+      (rc = dev_alloc(&e->d_full_list, Npad)))
+    return rc;
+  e->P = P;
+  e->NW = NW;
+  e->n_waves = (int)n_waves;
+  e->n_tiles = (int)n_tiles;
+  e->Npad = Npad;
+  return kOk;
+}
+
+extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* scales,
+                             int32_t* rmax, double* maxll, double* msums, double* vscale,
+                             rsp_stream_t stream_) {
+  if (!e) return fail(kErrNull, "rsp_emc_estep: null plan");
+  if (!model_c || !scales || !rmax || !maxll || !msums || !vscale)
+    return fail(kErrNull, "rsp_emc_estep: null pointer");
+  if (e->N == 0 || !e->d_slots) return fail(kErrState, "rsp_emc_estep: frames not set");
+  RSP_HIP(hipSetDevice(e->device));
+  hipStream_t st = (hipStream_t)stream_;
+  int rc;
+  // batch size: tables of `batch` hypotheses (default: <= 2 GiB and <= 16384)
+  int64_t batch = e->opt_batch > 0 ? e->opt_batch
+                                   : std::min<int64_t>(16384, (2LL << 30) / (4LL * e->TS));
+  batch = std::max<int64_t>(1, std::min(batch, e->H));
+  if (batch != e->batch) {
+    if ((rc = dev_alloc(&e->d_tables, batch * e->TS))) return rc;
+    RSP_HIP(hipMemsetAsync(e->d_tables, 0, sizeof(float) * batch * e->TS, st));
+    e->batch = batch;
+  }
+  // segments per batch: enough workgroups to fill 256 CUs several times
+  int nseg = (int)(e->opt_nseg > 0 ? e->opt_nseg : 0);
+  if (nseg == 0) {
+    const int64_t want = 2048;
+    nseg = (int)std::max<int64_t>(1, std::min<int64_t>(batch, ceil_div(want, e->n_tiles)));
+    if (nseg >= 8) nseg = nseg / 8 * 8;
+  }
+  nseg = (int)std::min<int64_t>(nseg, batch);
+  if ((int64_t)nseg * e->Npad > e->partial_cap) {
+    if ((rc = dev_alloc(&e->d_partials, (int64_t)nseg * e->Npad))) return rc;
+    e->partial_cap = (int64_t)nseg * e->Npad;
+  }
+  // 1-2
+  hipLaunchKernelGGL(k_holo_states, dim3(e->S), dim3(256), 0, st, model_c, e->d_sx, e->d_sy,
+                     e->d_dia, e->d_bg, e->d_probmask, e->size, e->d_I, msums);
+  RSP_LAUNCH_CHECK("k_holo_states");
+  hipLaunchKernelGGL(k_vscale_init, dim3(1), dim3(256), 0, st, msums, e->S, e->powder_sel_sum,
+                     e->d_init, e->d_vscale);
+  RSP_LAUNCH_CHECK("k_vscale_init");
+  hipLaunchKernelGGL(k_scatter_scale, dim3((unsigned)ceil_div(e->Npad, 256)), dim3(256), 0, st,
+                     scales, e->d_perm, (int)e->Npad, e->d_scale_sorted);
+  RSP_LAUNCH_CHECK("k_scatter_scale");
+  RSP_HIP(hipMemsetAsync(e->d_errmax, 0, sizeof(unsigned) * e->n_scored, st));
+  RSP_HIP(hipMemsetAsync(e->d_counters, 0, sizeof(unsigned) * 4, st));
+  GatherFn g = pick_gather(e->P, e->NW);
+  if (!g) return fail(kErrUnsupported, "no gather instantiation");
+  const size_t lds_bytes = sizeof(float) * (size_t)e->PSTRIDE;
+  RSP_HIP(hipFuncSetAttribute((const void*)g, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_bytes));
+  if (!e->exact_mode) {
+    const int HC = 64;
+    int first = 1;
+    const int64_t nbatch = ceil_div(e->H, batch);
+    if (e->timing) {
+      while ((int64_t)e->ev.size() < 4 * nbatch) {
+        hipEvent_t ev;
+        RSP_HIP(hipEventCreate(&ev));
+        e->ev.push_back(ev);
+      }
+    }
+    int64_t ib = 0;
+    for (int64_t hb = 0; hb < e->H; hb += batch, ++ib) {
+      const int nb = (int)std::min<int64_t>(batch, e->H - hb);
+      const int ns = std::min(nseg, nb);
+      dim3 tg((unsigned)ceil_div(e->n_scored, 256), (unsigned)ceil_div(nb, HC));
+      if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 0], st));
+      hipLaunchKernelGGL(k_tables, tg, dim3(256), 0, st, e->d_I, e->d_bg, e->d_scored_t,
+                         e->n_scored, e->size, e->R, e->d_rcos, e->d_rsin, (int)hb, nb, HC,
+                         e->PSTRIDE, e->TS, e->d_tables, e->d_errmax);
+      RSP_LAUNCH_CHECK("k_tables");
+      if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 1], st));
+      if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 2], st));
+      hipLaunchKernelGGL(g, dim3((unsigned)(ns * e->n_tiles)), dim3(64 * e->NW), lds_bytes, st,
+                         e->d_tables, e->TS, e->PSTRIDE, e->NP, (int)hb, nb, ns, e->n_tiles,
+                         e->d_slots, e->d_meta, e->d_scale_sorted, e->d_init, e->R,
+                         e->d_partials, (int)e->Npad);
+      RSP_LAUNCH_CHECK("k_gather");
+      if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));
+      hipLaunchKernelGGL(k_merge_top3, dim3((unsigned)ceil_div(e->Npad, 256)), dim3(256), 0, st,
+                         e->d_partials, ns, (int)e->Npad, e->d_running, first);
+      RSP_LAUNCH_CHECK("k_merge_top3");
+      first = 0;
+    }
+  }
+  ExactArgs A{e->d_I, e->d_bg, e->d_rcos, e->d_rsin, e->d_init, scales, e->d_ex_off,
+              e->d_ex_n1, e->d_ex_pix, e->d_ex_cnt, e->size, e->R};
+  hipLaunchKernelGGL(k_resolve, dim3((unsigned)ceil_div(e->Npad, 256)), dim3(256), 0, st, A,
+                     e->d_running, e->d_perm, (int)e->Npad, e->d_compact_of, e->d_errmax,
+                     e->exact_mode, rmax, maxll, e->d_full_list, e->d_counters);
+  RSP_LAUNCH_CHECK("k_resolve");
+  unsigned cnt[4] = {0, 0, 0, 0};
+  RSP_HIP(hipMemcpyAsync(cnt, e->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+  RSP_HIP(hipMemcpyAsync(vscale, e->d_vscale, sizeof(double), hipMemcpyDeviceToHost, st));
+  RSP_HIP(hipStreamSynchronize(st));
+  const int nfull = (int)cnt[2];
+  if (nfull > 0) {
+    const int nbx = (int)ceil_div(e->H, 256);
+    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(nfull, (64LL << 20) / nbx));
+    if ((int64_t)chunk * nbx > e->bcap) {
+      if ((rc = dev_alloc(&e->d_bval, (int64_t)chunk * nbx)) ||
+          (rc = dev_alloc(&e->d_bidx, (int64_t)chunk * nbx)))
+        return rc;
+      e->bcap = (int64_t)chunk * nbx;
+    }
+    for (int f0 = 0; f0 < nfull; f0 += chunk) {
+      const int nf = std::min(chunk, nfull - f0);
+      hipLaunchKernelGGL(k_full_search, dim3((unsigned)nbx, (unsigned)nf), dim3(256), 0, st, A,
+                         e->d_perm, e->d_full_list + f0, (int)e->H, e->d_bval, e->d_bidx, nbx);
+      RSP_LAUNCH_CHECK("k_full_search");
+      hipLaunchKernelGGL(k_full_reduce, dim3((unsigned)ceil_div(nf, 256)), dim3(256), 0, st,
+                         e->d_perm, e->d_full_list + f0, nf, e->d_bval, e->d_bidx, nbx, rmax,
+                         maxll);
+      RSP_LAUNCH_CHECK("k_full_reduce");
+    }
+    RSP_HIP(hipStreamSynchronize(st));
+  }
+  if (e->timing && !e->exact_mode) {
+    const int64_t nbatch = ceil_div(e->H, batch);
+    e->t_tables_ms = e->t_gather_ms = 0.;
+    for (int64_t ib = 0; ib < nbatch; ++ib) {
+      float a = 0.f, b = 0.f;
+      RSP_HIP(hipEventElapsedTime(&a, e->ev[4 * ib + 0], e->ev[4 * ib + 1]));
+      RSP_HIP(hipEventElapsedTime(&b, e->ev[4 * ib + 2], e->ev[4 * ib + 3]));
+      e->t_tables_ms += a;
+      e->t_gather_ms += b;
+    }
+    e->n_batches = nbatch;
+  }
+  e->stats[0] = e->N;
+  e->stats[1] = e->H;
+  e->stats[2] = cnt[0];
+  e->stats[3] = cnt[1];
+  e->stats[4] = cnt[2];
+  e->stats[5] = e->P;
+  e->stats[6] = e->NW;
+  e->stats[7] = e->NP;
+  return kOk;
+}
+
+extern "C" int rsp_emc_stats(const rsp_emc* e, int64_t* out) {
+  if (!e || !out) return fail(kErrNull, "rsp_emc_stats: null pointer");
+  for (int i = 0; i < 8; ++i) out[i] = e->stats[i];
+  return kOk;
+}
+
+extern "C" int rsp_emc_timing(const rsp_emc* e, double* out) {
+  if (!e || !out) return fail(kErrNull, "rsp_emc_timing: null pointer");
+  out[0] = e->t_tables_ms;
+  out[1] = e->t_gather_ms;
+  out[2] = (double)e->n_batches;
+  out[3] = (double)e->batch;
+  return kOk;
+}
+
+extern "C" int rsp_emc_destroy(rsp_emc* e) {
+  if (!e) return kOk;
+  hipSetDevice(e->device);
+  free_all(e);
+  delete e;
+  return kOk;
+}

@@ -1,0 +1,564 @@
+// MaxLP object update for gfx950: the replacement of maxLP.py:304-340 (_run_phaser).
+//
+// Reference path: for every model pixel t (rmask annulus first, then the full mask) a
+// sequential Python pattern search (maxLP.py:154-179) evaluates 33x33 complex candidates
+// f_c = f + pattern_c*step, each with LL_c = mean_d[K_d log w_dc] - mean_d[w_dc],
+// w_dc = rescale*|f_c + R_d(t)|^2 over all N frames (maxLP.py:181-197), K_d(t) being the
+// photon count of frame d at the detector pixel rint(R(-a_d) q_t size + c)
+// (maxLP.py:245-253).
+//
+// Device form (all pixels of a phase in parallel, one workgroup per pixel):
+//   * model-space photon lists: every detector photon (d, p, K) is pulled back to the
+//     model pixels t whose detector index is p (checked with the exact forward formula
+//     in a 5x5 window), giving per pixel the frames with K_d(t) > 0.  Counting, scan and
+//     a radix sort by (t, d) make the lists deterministic.
+//   * mean_d w_dc = rescale*(|f_c|^2 + 2 Re(f_c conj(m1)) + m2) from the per-pixel moments
+//     m1 = mean_d R_d(t), m2 = mean_d |R_d(t)|^2 (exact in real arithmetic).
+//   * sum_d K_d log|f_c + R_d|^2 = log prod_d |f_c + R_d|^(2K_d): a running FP64 product
+//     with periodic frexp renormalisation, one log per candidate at the end.
+//   * 128 threads x 9 candidates per workgroup; first-occurrence argmax (NaN first, as
+//     NumPy/CuPy argmax) by wave shuffles and LDS.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "refspi_amd.h"
+#include "rsp_common.hpp"
+
+namespace rsp {
+
+constexpr int kCand = 1089;       // 33 x 33 pattern (maxLP.py:45-48, nmax = 8)
+constexpr int kCentre = 544;      // pattern index of 0 + 0j
+constexpr int kThreads = 128;
+constexpr int kPerThread = 9;     // 128 * 9 = 1152 >= 1089
+constexpr int kChunk = 512;       // entries staged in LDS at a time
+
+struct Entry {
+  double rr, ri, k;
+};
+
+struct Geo {
+  int size, npix;
+  const double* qx;
+  const double* qy;
+  const double* qrad;
+};
+
+// maxLP.py:250-252 (+ clamp of kernels.cu:280-281 for even sizes; see the oracle).
+__device__ __forceinline__ int detector_index(const Geo& g, int t, double c, double s) {
+  const double qx = g.qx[t], qy = g.qy[t];
+  const double px = (c * qx + (-s) * qy) * g.size + (double)(g.size / 2);
+  const double py = (s * qx + c * qy) * g.size + (double)(g.size / 2);
+  int x = (int)rint(px), y = (int)rint(py);
+  x = min(max(x, 0), g.size - 1);
+  y = min(max(y, 0), g.size - 1);
+  return x * g.size + y;
+}
+
+// maxLP.py:238-243 get_fref_d for one (pixel, frame)
+__device__ __forceinline__ void fref(const Geo& g, int t, double dia, double sx, double sy,
+                                     double& rr, double& ri) {
+  const double pi = M_PI;
+  const double sval = pi * g.qrad[t] * dia;
+  const double f = (sin(sval) - sval * cos(sval)) / pow(sval, 3.);
+  const double theta = (2. * pi) * (g.qx[t] * sx + g.qy[t] * sy);
+  double sn, cs;
+  sincos(theta, &sn, &cs);
+  rr = f * cs;
+  ri = f * sn;
+}
+
+// Pull-back of detector photons to model pixels.  Photon e belongs to frame fr[e] at
+// detector pixel pix[e] with count cnt[e].  mode 0: count matches; mode 1: write keys.
+__global__ __launch_bounds__(256) void k_pull(Geo g, const int32_t* __restrict__ mask,
+                                              int64_t nph, const int32_t* __restrict__ fr,
+                                              const int32_t* __restrict__ pix,
+                                              const int32_t* __restrict__ cnt,
+                                              const double* __restrict__ rc,
+                                              const double* __restrict__ rs, int mode,
+                                              int32_t* __restrict__ nmatch,
+                                              const int64_t* __restrict__ pos,
+                                              uint64_t* __restrict__ keys) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nph) return;
+  const int d = fr[e], p = pix[e];
+  const double c = rc[d], s = rs[d];
+  const int c0 = g.size / 2;
+  const double X = (double)(p / g.size - c0), Y = (double)(p % g.size - c0);
+  // inverse of [[c, -s], [s, c]]
+  const double u = c * X + s * Y + c0, v = -s * X + c * Y + c0;
+  const int u0 = (int)rint(u), v0 = (int)rint(v);
+  int n = 0;
+  int64_t w = mode ? pos[e] : 0;
+  for (int du = -2; du <= 2; ++du)
+    for (int dv = -2; dv <= 2; ++dv) {
+      const int uu = u0 + du, vv = v0 + dv;
+      if (uu < 0 || uu >= g.size || vv < 0 || vv >= g.size) continue;
+      const int t = uu * g.size + vv;
+      if (!mask[t]) continue;
+      if (detector_index(g, t, c, s) != p) continue;
+      if (mode)
+        keys[w++] = ((uint64_t)t << 47) | ((uint64_t)d << 22) | (uint64_t)cnt[e];
+      ++n;
+    }
+  if (!mode) nmatch[e] = n;
+}
+
+__global__ __launch_bounds__(256) void k_bounds(const uint64_t* __restrict__ keys, int64_t n,
+                                                int64_t* __restrict__ seg_beg,
+                                                int64_t* __restrict__ seg_end) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int t = (int)(keys[i] >> 47);
+  if (i == 0 || (int)(keys[i - 1] >> 47) != t) seg_beg[t] = i;
+  if (i == n - 1 || (int)(keys[i + 1] >> 47) != t) seg_end[t] = i + 1;
+}
+
+__global__ __launch_bounds__(256) void k_entries(Geo g, const uint64_t* __restrict__ keys,
+                                                 int64_t n, const double* __restrict__ sx,
+                                                 const double* __restrict__ sy,
+                                                 const double* __restrict__ dia,
+                                                 Entry* __restrict__ ent) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t k = keys[i];
+  const int t = (int)(k >> 47);
+  const int d = (int)((k >> 22) & ((1u << 25) - 1));
+  const int K = (int)(k & ((1u << 22) - 1));
+  Entry e;
+  fref(g, t, dia[d], sx[d], sy[d], e.rr, e.ri);
+  e.k = (double)K;
+  ent[i] = e;
+}
+
+// Per-pixel moments m1 = mean_d R_d(t), m2 = mean_d |R_d(t)|^2 and K-bar = sum K / N.
+__global__ __launch_bounds__(256) void k_moments(Geo g, const int32_t* __restrict__ pixels,
+                                                 int N, const double* __restrict__ sx,
+                                                 const double* __restrict__ sy,
+                                                 const double* __restrict__ dia,
+                                                 double* __restrict__ mom) {
+  __shared__ double red[3][256];
+  const int t = pixels[blockIdx.x];
+  double a = 0., b = 0., q = 0.;
+  for (int d = threadIdx.x; d < N; d += 256) {
+    double rr, ri;
+    fref(g, t, dia[d], sx[d], sy[d], rr, ri);
+    a += rr;
+    b += ri;
+    q += rr * rr + ri * ri;
+  }
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b;
+  red[2][threadIdx.x] = q;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mom[4 * t + 0] = red[0][0] / N;
+    mom[4 * t + 1] = red[1][0] / N;
+    mom[4 * t + 2] = red[2][0] / N;
+  }
+}
+
+__device__ __forceinline__ bool cand_better(double a, int ia, double b, int ib) {
+  const bool na = isnan(a), nb = isnan(b);
+  if (na || nb) return na && (!nb || ia < ib);
+  return a > b || (a == b && ia < ib);
+}
+
+// One workgroup per pixel: up to 10 pattern-search steps (maxLP.py:154-179).
+// rescale < 0 selects the per-candidate rescale of phase 1 (maxLP.py:187).
+__global__ __launch_bounds__(kThreads) void k_pattern(
+    const int32_t* __restrict__ pixels, const int64_t* __restrict__ seg_beg,
+    const int64_t* __restrict__ seg_end, const Entry* __restrict__ ent,
+    const double* __restrict__ mom, int N, double rescale, double* __restrict__ model,
+    double* __restrict__ resc_out) {
+  __shared__ Entry sh[kChunk];
+  __shared__ double wv[kThreads / 64];
+  __shared__ int wi[kThreads / 64];
+  __shared__ double skt;
+  const int t = pixels[blockIdx.x];
+  const int64_t e0 = seg_beg[t], e1 = seg_end[t];
+  const double m1r = mom[4 * t + 0], m1i = mom[4 * t + 1], m2 = mom[4 * t + 2];
+  // K-bar = mean_d K_d (exact integer sum)
+  {
+    double ks = 0.;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kThreads) ks += ent[e].k;
+    for (int off = 32; off > 0; off >>= 1) ks += __shfl_xor(ks, off);
+    if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = ks;
+    __syncthreads();
+    if (threadIdx.x == 0) skt = (wv[0] + wv[1]) / N;
+    __syncthreads();
+  }
+  const double kbar = skt;
+  double fr = model[2 * t], fi = model[2 * t + 1];
+  double step = hypot(fr, fi) / 4.;
+  for (int it = 0; it < 10; ++it) {
+    double cr[kPerThread], ci[kPerThread], prod[kPerThread];
+    int ex[kPerThread];
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      const int c = threadIdx.x + kThreads * u;
+      const double px = -8. + 0.5 * (double)(c / 33), py = -8. + 0.5 * (double)(c % 33);
+      cr[u] = fr + px * step;
+      ci[u] = fi + py * step;
+      prod[u] = 1.;
+      ex[u] = 0;
+    }
+    for (int64_t cb = e0; cb < e1; cb += kChunk) {
+      const int n = (int)min((int64_t)kChunk, e1 - cb);
+      __syncthreads();
+      for (int i = threadIdx.x; i < n; i += kThreads) sh[i] = ent[cb + i];
+      __syncthreads();
+      for (int i = 0; i < n; ++i) {
+        const Entry E = sh[i];
+        const int K = (int)E.k;
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) {
+          const double re = cr[u] + E.rr, im = ci[u] + E.ri;
+          const double a = re * re + im * im;
+          double pr = prod[u] * a;
+          for (int k = 1; k < K; ++k) pr = pr * a;
+          prod[u] = pr;
+        }
+        if ((i & 3) == 3 || K > 1) {
+#pragma unroll
+          for (int u = 0; u < kPerThread; ++u) {
+            int e;
+            prod[u] = frexp(prod[u], &e);
+            ex[u] += e;
+          }
+        }
+      }
+    }
+    // candidate values and the local first-occurrence argmax
+    double bv = -INFINITY;
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      const int c = threadIdx.x + kThreads * u;
+      if (c >= kCand) continue;
+      const double mc = cr[u] * cr[u] + ci[u] * ci[u] + 2. * (cr[u] * m1r + ci[u] * m1i) + m2;
+      const double slog = (log(prod[u]) + (double)ex[u] * M_LN2) / N;
+      const double r = rescale < 0. ? kbar / mc : rescale;
+      const double v = kbar * log(r) + slog - r * mc;
+      if (cand_better(v, c, bv, bi)) {
+        bv = v;
+        bi = c;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (cand_better(ov, oi, bv, bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+      wv[threadIdx.x >> 6] = bv;
+      wi[threadIdx.x >> 6] = bi;
+    }
+    __syncthreads();
+    double v0 = wv[0];
+    int imax = wi[0];
+    if (cand_better(wv[1], wi[1], v0, imax)) imax = wi[1];
+    const double px = -8. + 0.5 * (double)(imax / 33), py = -8. + 0.5 * (double)(imax % 33);
+    const double nr = fr + px * step, ni = fi + py * step;
+    const double ns = imax == kCentre ? step / 6.6 : step;
+    if (ns / hypot(nr, ni) < 1.e-3) break;
+    fr = nr;
+    fi = ni;
+    step = ns;
+  }
+  if (threadIdx.x == 0) {
+    model[2 * t] = fr;
+    model[2 * t + 1] = fi;
+    if (resc_out) {
+      // maxLP.py:216-228 get_rescale_pixel at the final value
+      const double mc = fr * fr + fi * fi + 2. * (fr * m1r + fi * m1i) + m2;
+      resc_out[blockIdx.x] = kbar / mc;
+    }
+  }
+}
+
+}  // namespace rsp
+
+using namespace rsp;
+
+struct rsp_maxlp {
+  int device = 0, size = 0, npix = 0;
+  int64_t N = 0, nph = 0;
+  std::vector<int32_t> rmask_pix, mask_pix;
+  double *d_qx = nullptr, *d_qy = nullptr, *d_qrad = nullptr, *d_mom = nullptr,
+         *d_resc = nullptr;
+  int32_t *d_mask = nullptr, *d_rmask_pix = nullptr, *d_mask_pix = nullptr, *d_fr = nullptr,
+          *d_pix = nullptr, *d_cnt = nullptr, *d_nmatch = nullptr;
+  int64_t *d_pos = nullptr, *d_seg_beg = nullptr, *d_seg_end = nullptr;
+  uint64_t *d_keys = nullptr, *d_keys2 = nullptr;
+  Entry* d_ent = nullptr;
+  int64_t key_cap = 0;
+  void* d_tmp = nullptr;
+  size_t tmp_cap = 0;
+  double last_rescale = 0.;
+  int64_t n_entries = 0;
+};
+
+namespace {
+
+template <typename T>
+int alloc(T** p, int64_t n) {
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  return kOk;
+}
+
+template <typename T>
+int upload(T** p, const std::vector<T>& v) {
+  int rc = alloc(p, (int64_t)v.size());
+  if (rc) return rc;
+  if (!v.empty()) RSP_HIP(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+int ensure_tmp(rsp_maxlp* m, size_t bytes) {
+  if (bytes <= m->tmp_cap) return kOk;
+  if (m->d_tmp) hipFree(m->d_tmp);
+  m->d_tmp = nullptr;
+  RSP_HIP(hipMalloc(&m->d_tmp, bytes));
+  m->tmp_cap = bytes;
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" int rsp_maxlp_create(int device, int64_t size, rsp_maxlp** out) {
+  if (!out) return fail(kErrNull, "rsp_maxlp_create: null out");
+  *out = nullptr;
+  if (size < 8 || size > 362) return fail(kErrSize, "rsp_maxlp_create: size out of range");
+  RSP_HIP(hipSetDevice(device));
+  rsp_maxlp* m = new rsp_maxlp();
+  m->device = device;
+  m->size = (int)size;
+  m->npix = (int)(size * size);
+  // maxLP.py:79-95
+  std::vector<double> qx(m->npix), qy(m->npix), qrad(m->npix);
+  std::vector<int32_t> mask(m->npix);
+  const int c = m->size / 2;
+  for (int x = 0; x < m->size; ++x)
+    for (int y = 0; y < m->size; ++y) {
+      const int t = x * m->size + y;
+      qx[t] = (double)(x - c) / (double)m->size;
+      qy[t] = (double)(y - c) / (double)m->size;
+      double q = std::sqrt(qx[t] * qx[t] + qy[t] * qy[t]);
+      if (q == 0.) q = 1.e-6;
+      qrad[t] = q;
+      const bool mk = (q > 4. / m->size) && (q < 0.5);
+      const bool rk = (q > 0.39) && (q < 0.41);
+      mask[t] = mk;
+      if (mk) m->mask_pix.push_back(t);
+      if (rk) m->rmask_pix.push_back(t);
+    }
+  int rc;
+  if ((rc = upload(&m->d_qx, qx)) || (rc = upload(&m->d_qy, qy)) ||
+      (rc = upload(&m->d_qrad, qrad)) || (rc = upload(&m->d_mask, mask)) ||
+      (rc = upload(&m->d_rmask_pix, m->rmask_pix)) || (rc = upload(&m->d_mask_pix, m->mask_pix)) ||
+      (rc = alloc(&m->d_mom, 4LL * m->npix)) || (rc = alloc(&m->d_seg_beg, m->npix)) ||
+      (rc = alloc(&m->d_seg_end, m->npix)) ||
+      (rc = alloc(&m->d_resc, std::max<int64_t>(1, (int64_t)m->rmask_pix.size())))) {
+    delete m;
+    return rc;
+  }
+  *out = m;
+  return kOk;
+}
+
+extern "C" int rsp_maxlp_set_frames(rsp_maxlp* m, int64_t num_data, const int32_t* ones,
+                                    const int32_t* multi, const int32_t* place_ones,
+                                    const int32_t* place_multi, const int32_t* count_multi) {
+  if (!m || !ones || !multi) return fail(kErrNull, "rsp_maxlp_set_frames: null pointer");
+  if (num_data < 1 || num_data >= (1LL << 25))
+    return fail(kErrSize, "rsp_maxlp_set_frames: num_data out of range");
+  RSP_HIP(hipSetDevice(m->device));
+  std::vector<int32_t> fr, pix, cnt;
+  int64_t po = 0, pm = 0;
+  for (int64_t d = 0; d < num_data; ++d) {
+    if (ones[d] < 0 || multi[d] < 0) return fail(kErrSize, "negative photon count");
+    for (int k = 0; k < ones[d]; ++k) {
+      const int p = place_ones[po + k];
+      if (p < 0 || p >= m->npix) return fail(kErrSize, "place_ones out of range");
+      fr.push_back((int32_t)d);
+      pix.push_back(p);
+      cnt.push_back(1);
+    }
+    for (int k = 0; k < multi[d]; ++k) {
+      const int p = place_multi[pm + k];
+      const int cm = count_multi[pm + k];
+      if (p < 0 || p >= m->npix) return fail(kErrSize, "place_multi out of range");
+      if (cm < 1 || cm >= (1 << 22)) return fail(kErrSize, "count_multi out of range");
+      fr.push_back((int32_t)d);
+      pix.push_back(p);
+      cnt.push_back(cm);
+    }
+    po += ones[d];
+    pm += multi[d];
+  }
+  int rc;
+  if ((rc = upload(&m->d_fr, fr)) || (rc = upload(&m->d_pix, pix)) || (rc = upload(&m->d_cnt, cnt)) ||
+      (rc = alloc(&m->d_nmatch, (int64_t)fr.size())) || (rc = alloc(&m->d_pos, (int64_t)fr.size() + 1)))
+    return rc;
+  m->N = num_data;
+  m->nph = (int64_t)fr.size();
+  return kOk;
+}
+
+extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, const double* sy,
+                             const double* dia, const double* rot_cos, const double* rot_sin,
+                             double* rescale_out, rsp_stream_t stream_) {
+  if (!m || !model_c || !sx || !sy || !dia || !rot_cos || !rot_sin)
+    return fail(kErrNull, "rsp_maxlp_run: null pointer");
+  if (m->N == 0) return fail(kErrState, "rsp_maxlp_run: frames not set");
+  RSP_HIP(hipSetDevice(m->device));
+  hipStream_t st = (hipStream_t)stream_;
+  Geo g{m->size, m->npix, m->d_qx, m->d_qy, m->d_qrad};
+  int rc;
+  // model-space photon lists
+  int64_t total = 0;
+  if (m->nph > 0) {
+    hipLaunchKernelGGL(k_pull, dim3((unsigned)ceil_div(m->nph, 256)), dim3(256), 0, st, g,
+                       m->d_mask, m->nph, m->d_fr, m->d_pix, m->d_cnt, rot_cos, rot_sin, 0,
+                       m->d_nmatch, (const int64_t*)nullptr, (uint64_t*)nullptr);
+    RSP_LAUNCH_CHECK("k_pull(count)");
+    size_t tb = 0;
+    RSP_HIP(rocprim::exclusive_scan(nullptr, tb, m->d_nmatch, m->d_pos, (int64_t)0,
+                                    (size_t)m->nph, rocprim::plus<int64_t>(), st));
+    if ((rc = ensure_tmp(m, tb))) return rc;
+    RSP_HIP(rocprim::exclusive_scan(m->d_tmp, tb, m->d_nmatch, m->d_pos, (int64_t)0,
+                                    (size_t)m->nph, rocprim::plus<int64_t>(), st));
+    int64_t last_pos = 0;
+    int32_t last_n = 0;
+    RSP_HIP(hipMemcpyAsync(&last_pos, m->d_pos + (m->nph - 1), sizeof(int64_t),
+                           hipMemcpyDeviceToHost, st));
+    RSP_HIP(hipMemcpyAsync(&last_n, m->d_nmatch + (m->nph - 1), sizeof(int32_t),
+                           hipMemcpyDeviceToHost, st));
+    RSP_HIP(hipStreamSynchronize(st));
+    total = last_pos + last_n;
+  }
+  m->n_entries = total;
+  if (total > m->key_cap) {
+    if ((rc = alloc(&m->d_keys, total)) || (rc = alloc(&m->d_keys2, total)) ||
+        (rc = alloc(&m->d_ent, total)))
+      return rc;
+    m->key_cap = total;
+  }
+  RSP_HIP(hipMemsetAsync(m->d_seg_beg, 0, sizeof(int64_t) * m->npix, st));
+  RSP_HIP(hipMemsetAsync(m->d_seg_end, 0, sizeof(int64_t) * m->npix, st));
+  if (total > 0) {
+    hipLaunchKernelGGL(k_pull, dim3((unsigned)ceil_div(m->nph, 256)), dim3(256), 0, st, g,
+                       m->d_mask, m->nph, m->d_fr, m->d_pix, m->d_cnt, rot_cos, rot_sin, 1,
+                       m->d_nmatch, (const int64_t*)m->d_pos, m->d_keys);
+    RSP_LAUNCH_CHECK("k_pull(fill)");
+    size_t tb = 0;
+    RSP_HIP(rocprim::radix_sort_keys(nullptr, tb, m->d_keys, m->d_keys2, (size_t)total, 0, 64, st));
+    if ((rc = ensure_tmp(m, tb))) return rc;
+    RSP_HIP(rocprim::radix_sort_keys(m->d_tmp, tb, m->d_keys, m->d_keys2, (size_t)total, 0, 64, st));
+    hipLaunchKernelGGL(k_bounds, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st,
+                       m->d_keys2, total, m->d_seg_beg, m->d_seg_end);
+    RSP_LAUNCH_CHECK("k_bounds");
+    hipLaunchKernelGGL(k_entries, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, st, g,
+                       m->d_keys2, total, sx, sy, dia, m->d_ent);
+    RSP_LAUNCH_CHECK("k_entries");
+  }
+  const int nmask = (int)m->mask_pix.size(), nr = (int)m->rmask_pix.size();
+  if (nmask > 0) {
+    hipLaunchKernelGGL(k_moments, dim3(nmask), dim3(256), 0, st, g, m->d_mask_pix, (int)m->N, sx,
+                       sy, dia, m->d_mom);
+    RSP_LAUNCH_CHECK("k_moments");
+  }
+  // phase 1: rmask annulus with per-candidate rescale (maxLP.py:317-320)
+  if (nr > 0) {
+    hipLaunchKernelGGL(k_pattern, dim3(nr), dim3(kThreads), 0, st, m->d_rmask_pix, m->d_seg_beg,
+                       m->d_seg_end, m->d_ent, m->d_mom, (int)m->N, -1., model_c, m->d_resc);
+    RSP_LAUNCH_CHECK("k_pattern(phase1)");
+  }
+  // phase 2: mean rescale over the annulus (maxLP.py:324-329)
+  double rescale = 0.;
+  if (nr > 0) {
+    std::vector<double> r(nr);
+    RSP_HIP(hipMemcpyAsync(r.data(), m->d_resc, sizeof(double) * nr, hipMemcpyDeviceToHost, st));
+    RSP_HIP(hipStreamSynchronize(st));
+    // pairwise summation, as numpy.mean does for a contiguous float64 array
+    std::vector<double> buf(r);
+    double s = 0.;
+    // numpy's pairwise sum: blocks of 8 accumulators up to 128 elements, recursive halves
+    struct PW {
+      static double sum(const double* a, int64_t n) {
+        if (n < 8) {
+          double res = 0.;
+          for (int64_t i = 0; i < n; ++i) res += a[i];
+          return res;
+        }
+        if (n <= 128) {
+          double r8[8];
+          for (int k = 0; k < 8; ++k) r8[k] = a[k];
+          int64_t i;
+          for (i = 8; i < n - (n % 8); i += 8)
+            for (int k = 0; k < 8; ++k) r8[k] += a[i + k];
+          double res = ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
+          for (; i < n; ++i) res += a[i];
+          return res;
+        }
+        int64_t n2 = n / 2;
+        n2 -= n2 % 8;
+        return sum(a, n2) + sum(a + n2, n - n2);
+      }
+    };
+    s = PW::sum(buf.data(), nr);
+    rescale = s / (double)nr;
+  }
+  m->last_rescale = rescale;
+  if (rescale_out) *rescale_out = rescale;
+  // phase 3: full mask with the global rescale (maxLP.py:332-336)
+  if (nmask > 0) {
+    hipLaunchKernelGGL(k_pattern, dim3(nmask), dim3(kThreads), 0, st, m->d_mask_pix,
+                       m->d_seg_beg, m->d_seg_end, m->d_ent, m->d_mom, (int)m->N, rescale,
+                       model_c, (double*)nullptr);
+    RSP_LAUNCH_CHECK("k_pattern(phase3)");
+  }
+  RSP_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+extern "C" int rsp_maxlp_info(const rsp_maxlp* m, int64_t* out) {
+  if (!m || !out) return fail(kErrNull, "rsp_maxlp_info: null pointer");
+  out[0] = m->N;
+  out[1] = m->nph;
+  out[2] = m->n_entries;
+  out[3] = (int64_t)m->rmask_pix.size();
+  out[4] = (int64_t)m->mask_pix.size();
+  return kOk;
+}
+
+extern "C" int rsp_maxlp_destroy(rsp_maxlp* m) {
+  if (!m) return kOk;
+  hipSetDevice(m->device);
+  void* ptrs[] = {m->d_qx, m->d_qy, m->d_qrad, m->d_mom, m->d_resc, m->d_mask, m->d_rmask_pix,
+                  m->d_mask_pix, m->d_fr, m->d_pix, m->d_cnt, m->d_nmatch, m->d_pos,
+                  m->d_seg_beg, m->d_seg_end, m->d_keys, m->d_keys2, m->d_ent, m->d_tmp};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  delete m;
+  return kOk;
+}

@@ -1,0 +1,231 @@
+"""Seeded synthetic holographic SPI frames in the reference's sparse-photon layout.
+
+The workload recipe (Gaussian-blob object, spherical reference, in-plane rotation,
+Poisson sampling) follows the reference generator's forward model (make_data.py:199-333:
+hologram -> beamstop/edge mask -> normalise to the mean count -> Poisson -> sparse
+``place_ones``/``place_multi``/``count_multi``), with the object and fluence recipes of
+the benchmark workloads:
+
+* object: ``n_blobs`` Gaussian blobs (sigma 1.5 px, amplitudes U(0.5, 1)) at uniform
+  positions in a disk of ``disk_radius`` px inside a ``2*size`` px real-space box; its
+  Fourier transform is evaluated analytically on the central ``size x size`` q-grid
+  (identical to FFT-then-crop up to the negligible aliasing of a 1.5 px Gaussian);
+* rotation: the object is rotated by U[0, 2pi) per frame (F_obj(Rq));
+* reference: a uniform sphere, diameter ~ N(dia_mean, dia_sd) px and shift
+  U(-shift_max, shift_max) px per axis, in slice_gen_holo's convention
+  (kernels.cu:58-69), scaled to carry ``ref_factor`` times the object's photons;
+* fluence: constant, or log-normal with unit mean and the given sigma.
+
+Two back ends share the recipe: NumPy (small parity inputs, bit-reproducible across
+machines for a given seed) and torch (large inputs on the GPU, its own random stream).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class SynthConfig:
+    size: int = 64
+    num_data: int = 2000
+    mean_count: float = 150.
+    n_blobs: int = 40
+    disk_radius: float = 12.
+    sigma: float = 1.5
+    beamstop: float = 3.
+    dia_mean: float = 8.
+    dia_sd: float = 0.4
+    shift_max: float = 4.
+    ref_factor: float = 5.
+    fluence: str = 'constant'          # or 'lognormal'
+    fluence_sigma: float = 0.5
+    seed: int = 0
+
+    @property
+    def box(self):
+        return 2 * self.size
+
+
+# Benchmark workloads (frames x q-grid x rotations); see the bench and tests.
+CONFIG_CPU64 = SynthConfig()
+CONFIG_185 = SynthConfig(size=185, num_data=100_000, mean_count=300., n_blobs=120,
+                         disk_radius=30., beamstop=4.)
+CONFIG_256 = SynthConfig(size=256, num_data=200_000, mean_count=400., n_blobs=120,
+                         disk_radius=30., beamstop=4., fluence='lognormal',
+                         fluence_sigma=0.5)
+
+
+@dataclass
+class SparseFrames:
+    """Flat sparse-photon arrays (dset.py:23-60 layout) plus generator truth."""
+    num_pix: int
+    ones: np.ndarray
+    multi: np.ndarray
+    place_ones: np.ndarray
+    place_multi: np.ndarray
+    count_multi: np.ndarray
+    bg: np.ndarray | None = None
+    truth: dict = field(default_factory=dict)
+
+    @property
+    def num_data(self):
+        return len(self.ones)
+
+
+def _blobs(cfg: SynthConfig, rng):
+    """Blob centres (relative to the box centre) and amplitudes."""
+    n = cfg.n_blobs
+    r = cfg.disk_radius * np.sqrt(rng.random(n))
+    phi = 2 * np.pi * rng.random(n)
+    cen = np.stack([r * np.cos(phi), r * np.sin(phi)], 1)
+    amp = 0.5 + 0.5 * rng.random(n)
+    return cen, amp
+
+
+def _grid(size):
+    x = np.arange(size, dtype='f8') - size // 2
+    return x
+
+
+def _reference_ft(cfg: SynthConfig, dia, shx, shy):
+    """Sphere FT times shift ramp in slice_gen_holo's convention (kernels.cu:58-69)."""
+    size = cfg.size
+    g = _grid(size)
+    xx, yy = np.meshgrid(g, g, indexing='ij')
+    rad = np.sqrt(xx * xx + yy * yy)
+    s = rad[None] * np.pi * dia[:, None, None] / size
+    s[s == 0] = 1.e-5
+    sph = (np.sin(s) - s * np.cos(s)) / (s * s * s)
+    phase = 2 * np.pi * (xx[None] * shx[:, None, None] + yy[None] * shy[:, None, None]) / size
+    return sph * np.exp(1j * phase)
+
+
+def detector_mask(size, beamstop):
+    g = _grid(size)
+    xx, yy = np.meshgrid(g, g, indexing='ij')
+    rad = np.sqrt(xx * xx + yy * yy)
+    return ((rad >= beamstop) & (rad < size // 2)).astype('f8')
+
+
+def make_frames(cfg: SynthConfig, batch: int = 256) -> SparseFrames:
+    """NumPy generator (deterministic for a given seed)."""
+    rng = np.random.default_rng(cfg.seed)
+    size, n = cfg.size, cfg.num_data
+    cen, amp = _blobs(cfg, rng)
+    angles = 2 * np.pi * rng.random(n)
+    dia = cfg.dia_mean + cfg.dia_sd * rng.standard_normal(n)
+    shifts = rng.uniform(-cfg.shift_max, cfg.shift_max, (n, 2))
+    if cfg.fluence == 'lognormal':
+        fl = np.exp(cfg.fluence_sigma * rng.standard_normal(n) - 0.5 * cfg.fluence_sigma ** 2)
+    else:
+        fl = np.ones(n)
+    q = _grid(size) / cfg.box                       # cycles per real-space pixel
+    qq = q[:, None] ** 2 + q[None, :] ** 2
+    env = 2 * np.pi * cfg.sigma ** 2 * np.exp(-2 * np.pi ** 2 * cfg.sigma ** 2 * qq)
+    dmask = detector_mask(size, cfg.beamstop)
+    ones, multi, po, pm, cm = [], [], [], [], []
+    for b0 in range(0, n, batch):
+        b1 = min(n, b0 + batch)
+        ca, sa = np.cos(angles[b0:b1]), np.sin(angles[b0:b1])
+        # R^T c for each frame: centres rotated by -angle
+        u = ca[:, None] * cen[None, :, 0] + sa[:, None] * cen[None, :, 1]
+        v = -sa[:, None] * cen[None, :, 0] + ca[:, None] * cen[None, :, 1]
+        A = np.exp(-2j * np.pi * q[None, :, None] * u[:, None, :]) * amp[None, None, :]
+        B = np.exp(-2j * np.pi * q[None, :, None] * v[:, None, :])
+        fobj = env[None] * np.matmul(A, B.transpose(0, 2, 1))
+        fref = _reference_ft(cfg, dia[b0:b1], shifts[b0:b1, 0], shifts[b0:b1, 1])
+        pobj = (np.abs(fobj) ** 2 * dmask).sum((1, 2))
+        pref = (np.abs(fref) ** 2 * dmask).sum((1, 2))
+        rel = np.sqrt(cfg.ref_factor * pobj / pref)
+        inten = np.abs(fobj + rel[:, None, None] * fref) ** 2 * dmask
+        inten *= (cfg.mean_count * fl[b0:b1] / inten.sum((1, 2)))[:, None, None]
+        frames = rng.poisson(inten).reshape(b1 - b0, -1)
+        for f in frames:
+            o = np.nonzero(f == 1)[0]
+            m = np.nonzero(f > 1)[0]
+            ones.append(len(o))
+            multi.append(len(m))
+            po.append(o)
+            pm.append(m)
+            cm.append(f[m])
+    cat = lambda l: np.concatenate(l).astype('i4') if l else np.zeros(0, 'i4')
+    return SparseFrames(size * size, np.array(ones, 'i4'), np.array(multi, 'i4'), cat(po),
+                        cat(pm), cat(cm), None,
+                        dict(angles=angles, diameters=dia, shifts=shifts, fluence=fl,
+                             blob_centres=cen, blob_amps=amp))
+
+
+def make_frames_torch(cfg: SynthConfig, device='cuda', batch: int = 1024):
+    """torch generator for large workloads (float32 intensities, torch RNG stream).
+
+    Returns a dict of int32/int64 torch tensors on ``device`` in the flat sparse layout
+    (ones, multi, place_ones, place_multi, count_multi) plus truth tensors.
+    """
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(cfg.seed)
+    size, n = cfg.size, cfg.num_data
+    f32 = torch.float32
+    rng = np.random.default_rng(cfg.seed)
+    cen, amp = _blobs(cfg, rng)
+    cen = torch.tensor(cen, dtype=f32, device=device)
+    amp = torch.tensor(amp, dtype=f32, device=device)
+    angles = 2 * np.pi * torch.rand(n, generator=gen, device=device, dtype=torch.float64)
+    dia = cfg.dia_mean + cfg.dia_sd * torch.randn(n, generator=gen, device=device,
+                                                  dtype=torch.float64)
+    shifts = (torch.rand(n, 2, generator=gen, device=device, dtype=torch.float64) * 2 - 1) \
+        * cfg.shift_max
+    if cfg.fluence == 'lognormal':
+        fl = torch.exp(cfg.fluence_sigma * torch.randn(n, generator=gen, device=device,
+                                                       dtype=torch.float64)
+                       - 0.5 * cfg.fluence_sigma ** 2)
+    else:
+        fl = torch.ones(n, device=device, dtype=torch.float64)
+    g = torch.arange(size, device=device, dtype=f32) - size // 2
+    q = g / cfg.box
+    qq = q[:, None] ** 2 + q[None, :] ** 2
+    env = 2 * np.pi * cfg.sigma ** 2 * torch.exp(-2 * np.pi ** 2 * cfg.sigma ** 2 * qq)
+    xx, yy = torch.meshgrid(g, g, indexing='ij')
+    rad = torch.sqrt(xx * xx + yy * yy)
+    dmask = ((rad >= cfg.beamstop) & (rad < size // 2)).to(f32)
+    ones_l, multi_l, po_l, pm_l, cm_l = [], [], [], [], []
+    for b0 in range(0, n, batch):
+        b1 = min(n, b0 + batch)
+        a = angles[b0:b1].to(f32)
+        ca, sa = torch.cos(a), torch.sin(a)
+        u = ca[:, None] * cen[None, :, 0] + sa[:, None] * cen[None, :, 1]
+        v = -sa[:, None] * cen[None, :, 0] + ca[:, None] * cen[None, :, 1]
+        pa = -2 * np.pi * q[None, :, None] * u[:, None, :]
+        pb = -2 * np.pi * q[None, :, None] * v[:, None, :]
+        Ar, Ai = torch.cos(pa) * amp, torch.sin(pa) * amp
+        Br = torch.cos(pb).transpose(1, 2).contiguous()
+        Bi = torch.sin(pb).transpose(1, 2).contiguous()
+        fr = env * (torch.bmm(Ar, Br) - torch.bmm(Ai, Bi))
+        fi = env * (torch.bmm(Ar, Bi) + torch.bmm(Ai, Br))
+        d = dia[b0:b1].to(f32)[:, None, None]
+        s = rad[None] * np.pi * d / size
+        s = torch.where(s == 0, torch.full_like(s, 1.e-5), s)
+        sph = (torch.sin(s) - s * torch.cos(s)) / (s * s * s)
+        sh = shifts[b0:b1].to(f32)
+        ph = 2 * np.pi * (xx[None] * sh[:, 0, None, None] + yy[None] * sh[:, 1, None, None]) / size
+        rr, ri = sph * torch.cos(ph), sph * torch.sin(ph)
+        pobj = ((fr * fr + fi * fi) * dmask).sum((1, 2))
+        pref = ((rr * rr + ri * ri) * dmask).sum((1, 2))
+        rel = torch.sqrt(cfg.ref_factor * pobj / pref)[:, None, None]
+        inten = ((fr + rel * rr) ** 2 + (fi + rel * ri) ** 2) * dmask
+        inten = inten * (cfg.mean_count * fl[b0:b1].to(f32) / inten.sum((1, 2)))[:, None, None]
+        k = torch.poisson(inten, generator=gen).reshape(b1 - b0, -1).to(torch.int32)
+        one = k == 1
+        mul = k > 1
+        ones_l.append(one.sum(1, dtype=torch.int32))
+        multi_l.append(mul.sum(1, dtype=torch.int32))
+        po_l.append(torch.nonzero(one)[:, 1].to(torch.int32))
+        mi = torch.nonzero(mul)
+        pm_l.append(mi[:, 1].to(torch.int32))
+        cm_l.append(k[mi[:, 0], mi[:, 1]])
+    return dict(num_pix=size * size, ones=torch.cat(ones_l), multi=torch.cat(multi_l),
+                place_ones=torch.cat(po_l), place_multi=torch.cat(pm_l),
+                count_multi=torch.cat(cm_l), angles=angles, diameters=dia, shifts=shifts,
+                fluence=fl)

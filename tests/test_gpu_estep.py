@@ -1,0 +1,126 @@
+"""GPU parity of the E-step (C-ABI through the facade) against the FP64 oracle."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import SMALL, SMALL_GRID
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_setup(frames, model):
+    from oracle import emc_oracle as O
+    fr = O.SparseFramesNP(frames.ones, frames.multi, frames.place_ones, frames.place_multi,
+                          frames.count_multi, frames.num_pix)
+    g = SMALL_GRID
+    sx, sy, dia, ns = O.state_grid(g['shiftx'], g['shifty'], g['sphere_dia'])
+    return O, fr, (sx, sy, dia), ns
+
+
+def _emc(frames, model, **kw):
+    from refspi_amd.dset import Dataset
+    from refspi_amd.emc import EMC
+    g = SMALL_GRID
+    return EMC(dataset=Dataset(frames=frames), size=SMALL['size'], num_rot=g['num_rot'],
+               shiftx=g['shiftx'], shifty=g['shifty'], sphere_dia=g['sphere_dia'],
+               model=model, save=False, **kw)
+
+
+def test_mirror_slice_gen_holo_and_slice_gen(small_model):
+    import torch
+    from oracle import emc_oracle as O
+    from refspi_amd import _lib
+    size = SMALL['size']
+    bg = np.zeros(size * size)
+    m = torch.as_tensor(small_model, device='cuda')
+    tbg = torch.as_tensor(bg, device='cuda')
+    out = torch.empty(size * size, dtype=torch.float64, device='cuda')
+    for (sx, sy, dia, log) in ((0.5, -1.25, 7.3, 0), (-2., 2., 9., 1), (0., 0., 8., 0)):
+        _lib.call('rsp_slice_gen_holo', _lib.ptr(m), sx, sy, dia, 1., 1., size, _lib.ptr(tbg),
+                  log, _lib.ptr(out), _lib.stream_handle())
+        torch.cuda.synchronize()
+        ref = O.slice_gen_holo(small_model, sx, sy, dia, 1., 1., size, bg, log)
+        np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-13, atol=1e-13)
+    view = O.slice_gen_holo(small_model, 1., 0.5, 8., 1., 1., size, bg, 0)
+    tv = torch.as_tensor(view, device='cuda')
+    for ang in (0., 0.3, np.pi / 2, 2.1):
+        _lib.call('rsp_slice_gen', _lib.ptr(tv), ang, 1., size, _lib.ptr(tbg), 1,
+                  _lib.ptr(out), _lib.stream_handle())
+        torch.cuda.synchronize()
+        ref = O.slice_gen(view, ang, 1., size, bg, 1)
+        np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-13, atol=1e-12)
+
+
+def test_mirror_calc_prob_all(small_frames, small_model):
+    import torch
+    from refspi_amd import _lib
+    O, fr, states, ns = _oracle_setup(small_frames, small_model)
+    size = SMALL['size']
+    pm = O.probmask(size)
+    lv = O.slice_gen(O.slice_gen_holo(small_model, 1., -1., 8., 1., 1., size, fr.bg, 0),
+                     0.7, 1., size, fr.bg, 1)
+    scales = np.linspace(0.5, 1.5, fr.num_data)
+    ref = O.calc_prob_all(lv, pm, fr.ones, fr.multi, fr.ones_accum, fr.multi_accum,
+                          fr.place_ones, fr.place_multi, fr.count_multi, -3.5, scales)
+    d = lambda a: torch.as_tensor(np.ascontiguousarray(a), device='cuda')
+    out = torch.empty(fr.num_data, dtype=torch.float64, device='cuda')
+    args = [d(lv), d(pm), fr.num_data, d(fr.ones), d(fr.multi), d(fr.ones_accum),
+            d(fr.multi_accum), d(fr.place_ones), d(fr.place_multi), d(fr.count_multi)]
+    _lib.call('rsp_calc_prob_all', *[_lib.ptr(a) if not isinstance(a, int) else a for a in args],
+              -3.5, _lib.ptr(d(scales)), _lib.ptr(out), _lib.stream_handle())
+    torch.cuda.synchronize()
+    # same sequential order; only the log() implementations differ
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-13, atol=1e-11)
+
+
+@pytest.mark.parametrize('exact', [0, 1])
+def test_estep_argmax_matches_oracle(small_frames, small_model, exact):
+    O, fr, states, ns = _oracle_setup(small_frames, small_model)
+    prob, msums, vscale = O.estep_prob(small_model, SMALL['size'], states,
+                                       SMALL_GRID['num_rot'], fr, sequential=True)
+    mx, rmax = O.get_rmax(prob)
+    emc = _emc(small_frames, small_model)
+    if exact:
+        emc.set_option(1, 1)
+    import torch
+    emc._calculate_prob(torch.as_tensor(small_model, device='cuda'))
+    emc._get_rmax()
+    st = emc.estep_stats()
+    print('stats', st)
+    np.testing.assert_allclose(emc.msums, msums, rtol=1e-13)
+    assert abs(emc.vscale - vscale) <= 1e-13 * abs(vscale)
+    margin = O.top_margin(prob)
+    bad = np.nonzero(emc.rmax != rmax)[0]
+    # any disagreement must sit on an FP64 near-tie of the oracle itself
+    assert np.all(margin[bad] < 1e-9 * np.abs(mx[bad])), (bad, margin[bad])
+    assert len(bad) == 0
+    np.testing.assert_allclose(emc.maxll, mx, rtol=1e-12, atol=1e-9)
+    if not exact:
+        assert st['decided_fp32'] + st['top2_fp64'] + st['full_fp64'] == fr.num_data
+
+
+def test_estep_deterministic(small_frames, small_model):
+    import torch
+    emc = _emc(small_frames, small_model)
+    m = torch.as_tensor(small_model, device='cuda')
+    emc._calculate_prob(m)
+    r1, l1 = emc._get_rmax().copy(), emc.maxll.copy()
+    emc._calculate_prob(m)
+    r2, l2 = emc._get_rmax(), emc.maxll
+    assert np.array_equal(r1, r2)
+    assert np.array_equal(l1.view('u8'), l2.view('u8'))
+
+
+def test_estep_batching_and_segments_do_not_change_result(small_frames, small_model):
+    import torch
+    m = torch.as_tensor(small_model, device='cuda')
+    base = _emc(small_frames, small_model)
+    base._calculate_prob(m)
+    r0 = base._get_rmax().copy()
+    for batch, nseg in ((7, 3), (64, 8), (1000, 1)):
+        emc = _emc(small_frames, small_model)
+        emc.set_option(2, batch)
+        emc.set_option(4, nseg)
+        emc._calculate_prob(m)
+        assert np.array_equal(emc._get_rmax(), r0), (batch, nseg)
