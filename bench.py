@@ -83,7 +83,7 @@ def main():
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-frames', type=int, default=1000)
-    ap.add_argument('--cpu-states', type=int, default=3)
+    ap.add_argument('--cpu-states', type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
