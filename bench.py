@@ -84,7 +84,18 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-frames', type=int, default=1000)
     ap.add_argument('--cpu-states', type=int, default=10)
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only copy 1 GiB device-to-device (HBM counter calibration) and exit')
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device='cuda')
+        b = torch.ones(1 << 28, dtype=torch.float32, device='cuda')
+        torch.cuda.synchronize()
+        a.copy_(b)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_read': 1 << 30, 'copy_bytes_written': 1 << 30}))
+        return
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
               '(asked for --gpus %d)' % args.gpus, file=sys.stderr)
