@@ -81,6 +81,8 @@ def main():
     ap.add_argument('--frames', type=int, default=100_000)
     ap.add_argument('--num-rot', type=int, default=360)
     ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--banks', type=int, default=None,
+                    help='LDS bank model of the photon scheduler (0 = none, 32, 64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-frames', type=int, default=1000)
     ap.add_argument('--cpu-states', type=int, default=10)
@@ -108,7 +110,8 @@ def main():
     size = cfg.size
     t = time.time()
     emc = EMC(dataset=Dataset(frames=fr), size=size, num_rot=args.num_rot, shiftx=grid[0],
-              shifty=grid[1], sphere_dia=grid[2], seed=args.seed, save=False)
+              shifty=grid[1], sphere_dia=grid[2], seed=args.seed, save=False,
+              bank_count=args.banks)
     emc._make_plan()
     emc.set_option(5, 1)
     log('plans built in %.1f s' % (time.time() - t))

@@ -164,24 +164,43 @@ __global__ __launch_bounds__(256) void k_tables(
   atomicMax(errmax + i, __float_as_uint(emax));
 }
 
-// Compile-time walk over the slot registers in blocks of 8 so that every a[k] is a
-// statically named VGPR (a runtime-indexed loop would leave the array in scratch).
+// Compile-time walk over the slot registers in blocks of 16 so that every a[k] is a
+// statically named VGPR (a runtime-indexed loop would leave the array in scratch).  The
+// 16 LDS reads of a block are issued before the 16 FP64 additions that consume them.
 template <int K0, int P>
 __device__ __forceinline__ void gather_blocks(const char* lds, const uint32_t (&a)[P], int kbeg,
-                                              int kend, double& acc0, double& acc1) {
+                                              int kend, double (&acc)[4]) {
   if constexpr (K0 < P) {
     if (K0 >= kend) return;
     if (K0 >= kbeg) {
-      float v[8];
+      float v[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K0 + u]);
+      for (int u = 0; u < 16; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K0 + u]);
 #pragma unroll
-      for (int u = 0; u < 8; u += 2) {
-        acc0 += (double)v[u];
-        acc1 += (double)v[u + 1];
-      }
+      for (int u = 0; u < 16; ++u) acc[u & 3] += (double)v[u];
     }
-    gather_blocks<K0 + 8, P>(lds, a, kbeg, kend, acc0, acc1);
+    gather_blocks<K0 + 16, P>(lds, a, kbeg, kend, acc);
+  }
+}
+
+// native 4-float vector (HIP's float4 struct is copied through memcpy, which keeps a
+// register array of it in scratch)
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// prefetch buffer helpers (compile-time indices keep pf[] in VGPRs)
+template <int U, int PF, int T>
+__device__ __forceinline__ void pf_load(f4v (&pf)[PF], const f4v* __restrict__ src, int npf) {
+  if constexpr (U < PF) {
+    if (U < npf) pf[U] = src[threadIdx.x + U * T];
+    pf_load<U + 1, PF, T>(pf, src, npf);
+  }
+}
+
+template <int U, int PF, int T>
+__device__ __forceinline__ void pf_store(const f4v (&pf)[PF], f4v* lds4, int npf) {
+  if constexpr (U < PF) {
+    if (U < npf) lds4[threadIdx.x + U * T] = pf[U];
+    pf_store<U + 1, PF, T>(pf, lds4, npf);
   }
 }
 
@@ -199,14 +218,15 @@ __device__ __forceinline__ void load_slots(uint32_t (&a)[P], const uint32_t* __r
 // address (within the table part resident in LDS) of the frame's k-th photon entry;
 // wave_meta[gw*kNPMax + p] is the end slot of part p (a multiple of 8).
 // --------------------------------------------------------------------------------
-template <int P, int NW>
+template <int P, int NW, int PF>
 __global__ __launch_bounds__(64 * NW) void k_gather(
     const float* __restrict__ tables, int TS, int PSTRIDE, int NP, int hb, int nb,
-    int nseg, int n_tiles, const uint32_t* __restrict__ slots,
+    int nseg, int n_tiles, int gw0, const uint32_t* __restrict__ slots,
     const int32_t* __restrict__ wave_meta, const double* __restrict__ scale_sorted,
     const double* __restrict__ init, int R, Top3* __restrict__ partials, int Npad) {
-  extern __shared__ float4 lds4[];
+  extern __shared__ f4v lds4[];
   const char* lds = reinterpret_cast<const char*>(lds4);
+  constexpr int T = 64 * NW;
   const int b = blockIdx.x;
   int seg, tile;
   if ((nseg & 7) == 0) {  // keep one segment's workgroups on one XCD (round-robin dispatch)
@@ -218,35 +238,54 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
     tile = b % n_tiles;
   }
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int gw = tile * NW + wid;
+  const int lw = tile * NW + wid;  // wave within this slot class
+  const int gw = gw0 + lw;         // global wave
   const int j = gw * 64 + lane;
   const int h0 = hb + (int)((int64_t)seg * nb / nseg);
   const int h1 = hb + (int)((int64_t)(seg + 1) * nb / nseg);
 
   uint32_t a[P];
-  load_slots<0, P>(a, slots + (int64_t)gw * P * 64 + lane);
+  load_slots<0, P>(a, slots + (int64_t)lw * P * 64 + lane);
   int pend[kNPMax];
 #pragma unroll
   for (int p = 0; p < kNPMax; ++p) pend[p] = wave_meta[gw * kNPMax + p];
   const double scale = scale_sorted[j];
   Top3 top = top3_empty();
   const int n4 = PSTRIDE >> 2;
-
-  for (int h = h0; h < h1; ++h) {
-    const double base = init[h / R] * scale;
-    double acc0 = base, acc1 = 0.;
-    int kbeg = 0;
-    for (int p = 0; p < NP; ++p) {
-      const float4* src =
-          reinterpret_cast<const float4*>(tables + (int64_t)(h - hb) * TS + (int64_t)p * PSTRIDE);
-      __syncthreads();
-      for (int i = threadIdx.x; i < n4; i += 64 * NW) lds4[i] = src[i];
-      __syncthreads();
-      const int kend = pend[p];
-      gather_blocks<0, P>(lds, a, kbeg, kend, acc0, acc1);
-      kbeg = kend;
+  // the first npf*T float4 of each table part are prefetched into registers one step
+  // ahead (overlapping the gather); the short tail is copied synchronously
+  const int npf = min(PF, n4 / T);
+  const int tail0 = npf * T;
+  const int nq = (h1 - h0) * NP;  // steps over (hypothesis, part)
+  f4v pf[PF];
+  auto src_of = [&](int q) {
+    const int h = h0 + q / NP, p = q - (q / NP) * NP;
+    return reinterpret_cast<const f4v*>(tables + (int64_t)(h - hb) * TS + (int64_t)p * PSTRIDE);
+  };
+  if (nq > 0) pf_load<0, PF, T>(pf, src_of(0), npf);
+  double acc[4];
+  int kbeg = 0;
+  for (int q = 0; q < nq; ++q) {
+    const int h = h0 + q / NP, p = q - (q / NP) * NP;
+    if (p == 0) {
+      acc[0] = init[h / R] * scale;
+      acc[1] = acc[2] = acc[3] = 0.;
+      kbeg = 0;
     }
-    top3_insert(top, acc0 + acc1, h);
+    const f4v* src = src_of(q);
+    __syncthreads();
+    pf_store<0, PF, T>(pf, lds4, npf);
+#pragma unroll 4
+    for (int i = tail0 + threadIdx.x; i < n4; i += T) lds4[i] = src[i];
+    __syncthreads();
+    if (q + 1 < nq) pf_load<0, PF, T>(pf, src_of(q + 1), npf);
+    int kend = pend[0];
+#pragma unroll
+    for (int pp = 1; pp < kNPMax; ++pp)
+      if (p == pp) kend = pend[pp];
+    gather_blocks<0, P>(lds, a, kbeg, kend, acc);
+    kbeg = kend;
+    if (p == NP - 1) top3_insert(top, (acc[0] + acc[1]) + (acc[2] + acc[3]), h);
   }
   partials[(int64_t)seg * Npad + j] = top;
 }
@@ -455,6 +494,11 @@ struct rsp_emc {
   int exact_mode = 0;
   int64_t opt_batch = 0, opt_nseg = 0;
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  struct Class {
+    int cfg, n_tiles, gw0;
+    int64_t w_src, n_src, slot_off;
+  };
+  std::vector<Class> classes;
   // optional per-kernel timing with HIP events on the plan's stream
   int timing = 0;
   std::vector<hipEvent_t> ev;  // 4 per batch: tables begin/end, gather begin/end
@@ -513,6 +557,11 @@ int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
     max_len = std::max(max_len, (int)entries[l].size());
   }
   int nslots = 0;
+  if (nbanks == 0) {  // plain packing: entry r of every lane in slot r
+    for (int l = 0; l < G; ++l)
+      for (size_t r = 0; r < entries[l].size(); ++r) slot_of[l][r] = (int)r;
+    return max_len;
+  }
   auto lane_bit = [&](int l, int k) -> bool {
     const auto& w = lane_used[l];
     return (size_t)(k >> 6) < w.size() && ((w[k >> 6] >> (k & 63)) & 1ull);
@@ -537,36 +586,31 @@ int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
   return nslots;
 }
 
-int choose_P(int need) {
-  static const int Ps[] = {8, 16, 32, 48, 64, 96, 128, 160, 192};
-  for (int p : Ps)
-    if (need <= p) return p;
+using GatherFn = void (*)(const float*, int, int, int, int, int, int, int, int,
+                          const uint32_t*, const int32_t*, const double*, const double*, int,
+                          Top3*, int);
+
+// Gather instantiations: P slot registers per lane, NW waves per workgroup (so that the
+// P slots + ~40 working VGPRs + 4*PF prefetch VGPRs fit NW/4 waves per SIMD), PF float4
+// of the next table prefetched per thread.
+struct GatherCfg {
+  int P, NW, PF;
+  GatherFn fn;
+};
+
+const GatherCfg kGatherCfgs[] = {
+    {16, 16, 8, &k_gather<16, 16, 8>},   {32, 16, 8, &k_gather<32, 16, 8>},
+    {48, 16, 7, &k_gather<48, 16, 7>},   {64, 16, 3, &k_gather<64, 16, 3>},
+    {80, 12, 7, &k_gather<80, 12, 7>},   {96, 12, 5, &k_gather<96, 12, 5>},
+    {128, 8, 12, &k_gather<128, 8, 12>}, {160, 8, 10, &k_gather<160, 8, 10>},
+    {192, 8, 3, &k_gather<192, 8, 3>},
+};
+constexpr int kNumCfg = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
+
+int choose_cfg(int need) {
+  for (int i = 0; i < kNumCfg; ++i)
+    if (need <= kGatherCfgs[i].P) return i;
   return -1;
-}
-
-// waves per workgroup so that the P slot registers (+~38 working VGPRs) fit the
-// 512-entry VGPR file of a SIMD at NW/4 waves per SIMD
-int choose_NW(int P) {
-  if (P <= 64) return 16;
-  if (P <= 128) return 12;
-  return 8;
-}
-
-using GatherFn = void (*)(const float*, int, int, int, int, int, int, int, const uint32_t*,
-                          const int32_t*, const double*, const double*, int, Top3*, int);
-
-template <int P, int NW>
-GatherFn gfn() {
-  return &k_gather<P, NW>;
-}
-
-GatherFn pick_gather(int P, int NW) {
-#define RSP_G(p, nw) \
-  if (P == p && NW == nw) return gfn<p, nw>();
-  RSP_G(8, 16) RSP_G(16, 16) RSP_G(32, 16) RSP_G(48, 16) RSP_G(64, 16) RSP_G(96, 12)
-  RSP_G(128, 12) RSP_G(160, 8) RSP_G(192, 8)
-#undef RSP_G
-  return nullptr;
 }
 
 }  // namespace
@@ -647,7 +691,8 @@ extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
       e->opt_batch = value;
       return kOk;
     case 3:
-      if (value != 32 && value != 64) return fail(kErrSize, "bank count must be 32 or 64");
+      if (value != 0 && value != 32 && value != 64)
+        return fail(kErrSize, "bank count must be 0 (no scheduling), 32 or 64");
       e->NB = (int)value;
       return kOk;
     case 4:
@@ -730,7 +775,8 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return nentries[a] > nentries[b]; });
   const int NB = e->NB;
-  const int G = NB;  // lanes per conflict group: 32 banks -> half waves, 64 -> full waves
+  // lanes per conflict group: 32 banks -> half waves, 64 -> full waves, 0 -> no scheduling
+  const int G = NB == 0 ? 64 : NB;
   const int64_t nw0 = ceil_div(N, 64);
   // per wave, per part: slot assignments
   std::vector<std::vector<std::vector<int>>> wave_slots(nw0);  // [w][lane] -> entries by slot
@@ -765,12 +811,12 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
           for (size_t r = 0; r < ent[l].size(); ++r) pl[slot_of[l][r]] = ent[l][r];
         }
       }
-      part_slots = (int)ceil_div(part_slots, 8) * 8;
+      part_slots = (int)ceil_div(part_slots, 16) * 16;
       const int part_len = std::min(kPartMax, e->n_scored - p * kPartMax);
       for (int g0 = 0; g0 < 64; g0 += G) {
         for (int k = 0; k < part_slots; ++k) {
           uint64_t used = 0;
-          for (int l = 0; l < G; ++l) {
+          for (int l = 0; l < G && NB > 0; ++l) {
             auto& pl = placed[g0 + l];
             if (k < (int)pl.size() && pl[k] >= 0) used |= 1ull << (pl[k] % NB);
           }
@@ -778,6 +824,10 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
             auto& pl = placed[g0 + l];
             if ((int)pl.size() <= k) pl.resize(k + 1, -1);
             if (pl[k] >= 0) continue;
+            if (NB == 0) {
+              pl[k] = part_len + l;  // distinct zero pads
+              continue;
+            }
             int bank = 0;
             while (bank < NB && ((used >> bank) & 1ull)) ++bank;
             if (bank == NB) bank = 0;
@@ -795,27 +845,56 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
     wave_slots[w] = std::move(lane_slots);
     need = std::max(need, kbase);
   }
-  const int P = choose_P(std::max(need, 8));
-  if (P < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs more than 192 slots");
-  const int NW = choose_NW(P);
-  const int64_t n_tiles = ceil_div(nw0, NW);
-  const int64_t n_waves = n_tiles * NW;
+  // slot classes: each wave gets the smallest instantiation that holds its slots; the
+  // choice is made non-increasing along the (descending) wave order so that classes are
+  // contiguous runs of waves
+  std::vector<int> wcfg(nw0);
+  for (int64_t w = nw0 - 1; w >= 0; --w) {
+    int c = choose_cfg(std::max(wave_pend[w][kNPMax - 1], 16));
+    if (c < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs more than 192 slots");
+    if (w + 1 < nw0) c = std::max(c, wcfg[w + 1]);
+    wcfg[w] = c;
+  }
+  e->classes.clear();
+  int64_t gw_total = 0, slot_total = 0;
+  for (int64_t w = 0; w < nw0;) {
+    int64_t w1 = w;
+    while (w1 < nw0 && wcfg[w1] == wcfg[w]) ++w1;
+    rsp_emc::Class c;
+    c.cfg = wcfg[w];
+    const GatherCfg& g = kGatherCfgs[c.cfg];
+    c.w_src = w;
+    c.n_src = w1 - w;
+    c.n_tiles = (int)ceil_div(c.n_src, g.NW);
+    c.gw0 = (int)gw_total;
+    c.slot_off = slot_total;
+    gw_total += (int64_t)c.n_tiles * g.NW;
+    slot_total += (int64_t)c.n_tiles * g.NW * g.P * 64;
+    e->classes.push_back(c);
+    w = w1;
+  }
+  const int64_t n_waves = gw_total;
   const int64_t Npad = n_waves * 64;
-  std::vector<uint32_t> slots((size_t)(n_waves * P * 64));
-  std::vector<int32_t> meta((size_t)(n_waves * kNPMax));
+  std::vector<uint32_t> slots((size_t)slot_total);
+  std::vector<int32_t> meta((size_t)(n_waves * kNPMax), 0);
   std::vector<int32_t> perm(Npad, -1);
-  for (int64_t w = 0; w < n_waves; ++w) {
-    for (int p = 0; p < kNPMax; ++p) meta[w * kNPMax + p] = w < nw0 ? wave_pend[w][p] : 0;
-    for (int l = 0; l < 64; ++l) {
-      const int64_t jj = w * 64 + l;
-      if (jj < N) perm[jj] = order[jj];
-      for (int k = 0; k < P; ++k) {
-        int entry;
-        if (w < nw0 && k < (int)wave_slots[w][l].size())
-          entry = wave_slots[w][l][k];
-        else
-          entry = std::min(kPartMax, e->n_scored) + (l % 64);  // pad (never read if k >= end)
-        slots[((size_t)w * P + k) * 64 + l] = (uint32_t)entry * 4u;
+  const uint32_t never_read = (uint32_t)std::min(kPartMax, e->n_scored);
+  for (const auto& c : e->classes) {
+    const GatherCfg& g = kGatherCfgs[c.cfg];
+    for (int64_t lw = 0; lw < (int64_t)c.n_tiles * g.NW; ++lw) {
+      const int64_t gw = c.gw0 + lw, w = c.w_src + lw;
+      const bool real = lw < c.n_src;
+      for (int p = 0; p < kNPMax; ++p) meta[gw * kNPMax + p] = real ? wave_pend[w][p] : 0;
+      for (int l = 0; l < 64; ++l) {
+        if (real && w * 64 + l < N) perm[gw * 64 + l] = order[w * 64 + l];
+        for (int k = 0; k < g.P; ++k) {
+          int entry;
+          if (real && k < (int)wave_slots[w][l].size())
+            entry = wave_slots[w][l][k];
+          else
+            entry = (int)never_read + (l % 64);  // a zero pad (never read if k >= end)
+          slots[(size_t)(c.slot_off + ((int64_t)lw * g.P + k) * 64 + l)] = (uint32_t)entry * 4u;
+        }
       }
     }
   }
@@ -828,14 +907,12 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
       (rc = dev_alloc(&e->d_scale_sorted, Npad)) || (rc = dev_alloc(&e->d_running, Npad)) ||
       (rc = dev_alloc(&e->d_full_list, Npad)))
     return rc;
-  e->P = P;
-  e->NW = NW;
+  e->P = kGatherCfgs[e->classes.front().cfg].P;
+  e->NW = kGatherCfgs[e->classes.front().cfg].NW;
   e->n_waves = (int)n_waves;
-  e->n_tiles = (int)n_tiles;
   e->Npad = Npad;
   return kOk;
 }
-
 extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* scales,
                              int32_t* rmax, double* maxll, double* msums, double* vscale,
                              rsp_stream_t stream_) {
@@ -859,7 +936,9 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
   int nseg = (int)(e->opt_nseg > 0 ? e->opt_nseg : 0);
   if (nseg == 0) {
     const int64_t want = 2048;
-    nseg = (int)std::max<int64_t>(1, std::min<int64_t>(batch, ceil_div(want, e->n_tiles)));
+    int64_t tiles = 0;
+    for (const auto& c : e->classes) tiles += c.n_tiles;
+    nseg = (int)std::max<int64_t>(1, std::min<int64_t>(batch, ceil_div(want, tiles)));
     if (nseg >= 8) nseg = nseg / 8 * 8;
   }
   nseg = (int)std::min<int64_t>(nseg, batch);
@@ -879,11 +958,10 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
   RSP_LAUNCH_CHECK("k_scatter_scale");
   RSP_HIP(hipMemsetAsync(e->d_errmax, 0, sizeof(unsigned) * e->n_scored, st));
   RSP_HIP(hipMemsetAsync(e->d_counters, 0, sizeof(unsigned) * 4, st));
-  GatherFn g = pick_gather(e->P, e->NW);
-  if (!g) return fail(kErrUnsupported, "no gather instantiation");
   const size_t lds_bytes = sizeof(float) * (size_t)e->PSTRIDE;
-  RSP_HIP(hipFuncSetAttribute((const void*)g, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds_bytes));
+  for (const auto& c : e->classes)
+    RSP_HIP(hipFuncSetAttribute((const void*)kGatherCfgs[c.cfg].fn,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
   if (!e->exact_mode) {
     const int HC = 64;
     int first = 1;
@@ -907,10 +985,13 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
       RSP_LAUNCH_CHECK("k_tables");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 1], st));
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 2], st));
-      hipLaunchKernelGGL(g, dim3((unsigned)(ns * e->n_tiles)), dim3(64 * e->NW), lds_bytes, st,
-                         e->d_tables, e->TS, e->PSTRIDE, e->NP, (int)hb, nb, ns, e->n_tiles,
-                         e->d_slots, e->d_meta, e->d_scale_sorted, e->d_init, e->R,
-                         e->d_partials, (int)e->Npad);
+      for (const auto& c : e->classes) {
+        const GatherCfg& gc = kGatherCfgs[c.cfg];
+        hipLaunchKernelGGL(gc.fn, dim3((unsigned)(ns * c.n_tiles)), dim3(64 * gc.NW), lds_bytes,
+                           st, e->d_tables, e->TS, e->PSTRIDE, e->NP, (int)hb, nb, ns, c.n_tiles,
+                           c.gw0, e->d_slots + c.slot_off, e->d_meta, e->d_scale_sorted,
+                           e->d_init, e->R, e->d_partials, (int)e->Npad);
+      }
       RSP_LAUNCH_CHECK("k_gather");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));
       hipLaunchKernelGGL(k_merge_top3, dim3((unsigned)ceil_div(e->Npad, 256)), dim3(256), 0, st,

@@ -112,6 +112,19 @@ def test_estep_deterministic(small_frames, small_model):
     assert np.array_equal(l1.view('u8'), l2.view('u8'))
 
 
+@pytest.mark.parametrize('banks', [0, 64])
+def test_estep_bank_models_do_not_change_result(small_frames, small_model, banks):
+    import torch
+    m = torch.as_tensor(small_model, device='cuda')
+    base = _emc(small_frames, small_model)
+    base._calculate_prob(m)
+    r0 = base._get_rmax().copy()
+    emc = _emc(small_frames, small_model, bank_count=banks)
+    emc._calculate_prob(m)
+    assert np.array_equal(emc._get_rmax(), r0)
+    np.testing.assert_array_equal(emc.maxll, base.maxll)
+
+
 def test_estep_batching_and_segments_do_not_change_result(small_frames, small_model):
     import torch
     m = torch.as_tensor(small_model, device='cuda')
