@@ -223,7 +223,8 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
     const float* __restrict__ tables, int TS, int PSTRIDE, int NP, int hb, int nb,
     int nseg, int n_tiles, int gw0, const uint32_t* __restrict__ slots,
     const int32_t* __restrict__ wave_meta, const double* __restrict__ scale_sorted,
-    const double* __restrict__ init, int R, Top3* __restrict__ partials, int Npad) {
+    const double* __restrict__ init, int R, Top3* __restrict__ partials, int Npad,
+    int pf_cap) {
   extern __shared__ f4v lds4[];
   const char* lds = reinterpret_cast<const char*>(lds4);
   constexpr int T = 64 * NW;
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   const int n4 = PSTRIDE >> 2;
   // the first npf*T float4 of each table part are prefetched into registers one step
   // ahead (overlapping the gather); the short tail is copied synchronously
-  const int npf = min(PF, n4 / T);
+  const int npf = min(min(PF, n4 / T), pf_cap);
   const int tail0 = npf * T;
   const int nq = (h1 - h0) * NP;  // steps over (hypothesis, part)
   f4v pf[PF];
@@ -290,11 +291,23 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   partials[(int64_t)seg * Npad + j] = top;
 }
 
-__global__ __launch_bounds__(256) void k_merge_top3(const Top3* __restrict__ partials, int nseg,
+// per-class segment counts for the merge (waves [gw_end[c-1], gw_end[c]) use nseg[c])
+constexpr int kMaxClass = 16;
+struct SegMap {
+  int n;
+  int gw_end[kMaxClass];
+  int nseg[kMaxClass];
+};
+
+__global__ __launch_bounds__(256) void k_merge_top3(const Top3* __restrict__ partials, SegMap sm,
                                                     int Npad, Top3* __restrict__ running,
                                                     int first) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   if (j >= Npad) return;
+  const int gw = j >> 6;
+  int nseg = 0;
+  for (int c = sm.n - 1; c >= 0; --c)
+    if (gw < sm.gw_end[c]) nseg = sm.nseg[c];
   Top3 t = first ? top3_empty() : running[j];
   for (int g = 0; g < nseg; ++g) {
     const Top3 p = partials[(int64_t)g * Npad + j];
@@ -493,6 +506,7 @@ struct rsp_emc {
   // options / stats
   int exact_mode = 0;
   int64_t opt_batch = 0, opt_nseg = 0;
+  int pf_cap = 1 << 20;  // max float4 per thread prefetched into registers
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   struct Class {
     int cfg, n_tiles, gw0;
@@ -588,7 +602,7 @@ int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
 
 using GatherFn = void (*)(const float*, int, int, int, int, int, int, int, int,
                           const uint32_t*, const int32_t*, const double*, const double*, int,
-                          Top3*, int);
+                          Top3*, int, int);
 
 // Gather instantiations: P slot registers per lane, NW waves per workgroup (so that the
 // P slots + ~40 working VGPRs + 4*PF prefetch VGPRs fit NW/4 waves per SIMD), PF float4
@@ -700,6 +714,7 @@ extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
       e->opt_nseg = value;
       return kOk;
     case 5: e->timing = value != 0; return kOk;
+    case 6: e->pf_cap = value < 0 ? (1 << 20) : (int)value; return kOk;
     default: return fail(kErrSize, "rsp_emc_set_option: unknown key");
   }
 }
@@ -932,19 +947,21 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
     RSP_HIP(hipMemsetAsync(e->d_tables, 0, sizeof(float) * batch * e->TS, st));
     e->batch = batch;
   }
-  // segments per batch: enough workgroups to fill 256 CUs several times
-  int nseg = (int)(e->opt_nseg > 0 ? e->opt_nseg : 0);
-  if (nseg == 0) {
-    const int64_t want = 2048;
-    int64_t tiles = 0;
-    for (const auto& c : e->classes) tiles += c.n_tiles;
-    nseg = (int)std::max<int64_t>(1, std::min<int64_t>(batch, ceil_div(want, tiles)));
-    if (nseg >= 8) nseg = nseg / 8 * 8;
-  }
-  nseg = (int)std::min<int64_t>(nseg, batch);
-  if ((int64_t)nseg * e->Npad > e->partial_cap) {
-    if ((rc = dev_alloc(&e->d_partials, (int64_t)nseg * e->Npad))) return rc;
-    e->partial_cap = (int64_t)nseg * e->Npad;
+  // segments per batch and slot class: each class launch gets about 4 workgroups per CU
+  // (a class with few tiles gets more hypothesis segments), so that every launch fills the
+  // 256 CUs and the classes take time in proportion to their frames
+  if ((int)e->classes.size() > kMaxClass) return fail(kErrUnsupported, "too many slot classes");
+  auto nseg_of = [&](const rsp_emc::Class& c, int64_t nb) {
+    int64_t ns = e->opt_nseg > 0 ? e->opt_nseg : ceil_div(4 * 256, c.n_tiles);
+    ns = std::max<int64_t>(1, std::min<int64_t>(ns, nb));
+    if (ns >= 8 && e->opt_nseg == 0) ns = ns / 8 * 8;
+    return (int)ns;
+  };
+  int nseg_max = 1;
+  for (const auto& c : e->classes) nseg_max = std::max(nseg_max, nseg_of(c, batch));
+  if ((int64_t)nseg_max * e->Npad > e->partial_cap) {
+    if ((rc = dev_alloc(&e->d_partials, (int64_t)nseg_max * e->Npad))) return rc;
+    e->partial_cap = (int64_t)nseg_max * e->Npad;
   }
   // 1-2
   hipLaunchKernelGGL(k_holo_states, dim3(e->S), dim3(256), 0, st, model_c, e->d_sx, e->d_sy,
@@ -976,7 +993,8 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
     int64_t ib = 0;
     for (int64_t hb = 0; hb < e->H; hb += batch, ++ib) {
       const int nb = (int)std::min<int64_t>(batch, e->H - hb);
-      const int ns = std::min(nseg, nb);
+      SegMap sm;
+      sm.n = (int)e->classes.size();
       dim3 tg((unsigned)ceil_div(e->n_scored, 256), (unsigned)ceil_div(nb, HC));
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 0], st));
       hipLaunchKernelGGL(k_tables, tg, dim3(256), 0, st, e->d_I, e->d_bg, e->d_scored_t,
@@ -985,17 +1003,21 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
       RSP_LAUNCH_CHECK("k_tables");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 1], st));
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 2], st));
-      for (const auto& c : e->classes) {
+      for (int ci = 0; ci < sm.n; ++ci) {
+        const auto& c = e->classes[ci];
         const GatherCfg& gc = kGatherCfgs[c.cfg];
+        const int ns = nseg_of(c, nb);
+        sm.gw_end[ci] = c.gw0 + c.n_tiles * gc.NW;
+        sm.nseg[ci] = ns;
         hipLaunchKernelGGL(gc.fn, dim3((unsigned)(ns * c.n_tiles)), dim3(64 * gc.NW), lds_bytes,
                            st, e->d_tables, e->TS, e->PSTRIDE, e->NP, (int)hb, nb, ns, c.n_tiles,
                            c.gw0, e->d_slots + c.slot_off, e->d_meta, e->d_scale_sorted,
-                           e->d_init, e->R, e->d_partials, (int)e->Npad);
+                           e->d_init, e->R, e->d_partials, (int)e->Npad, e->pf_cap);
       }
       RSP_LAUNCH_CHECK("k_gather");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));
       hipLaunchKernelGGL(k_merge_top3, dim3((unsigned)ceil_div(e->Npad, 256)), dim3(256), 0, st,
-                         e->d_partials, ns, (int)e->Npad, e->d_running, first);
+                         e->d_partials, sm, (int)e->Npad, e->d_running, first);
       RSP_LAUNCH_CHECK("k_merge_top3");
       first = 0;
     }
