@@ -81,8 +81,6 @@ def main():
     ap.add_argument('--frames', type=int, default=100_000)
     ap.add_argument('--num-rot', type=int, default=360)
     ap.add_argument('--seed', type=int, default=0)
-    ap.add_argument('--prefetch', type=int, default=-1,
-                    help='cap on prefetched float4 per thread (-1 default, 0 off)')
     ap.add_argument('--banks', type=int, default=None,
                     help='LDS bank model of the photon scheduler (0 = none, 32, 64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
@@ -116,7 +114,6 @@ def main():
               bank_count=args.banks)
     emc._make_plan()
     emc.set_option(5, 1)
-    emc.set_option(6, args.prefetch)
     log('plans built in %.1f s' % (time.time() - t))
     S = len(emc.shiftx)
     H = S * args.num_rot

@@ -82,11 +82,10 @@ int rsp_emc_set_frames(rsp_emc* plan, int64_t num_data, const int32_t* ones,
 /* Tuning / mode switches.  Keys:
  *   1 = exact mode (1: every frame searched in FP64 over all hypotheses; default 0)
  *   2 = hypotheses per table batch (default chosen from memory)
- *   3 = LDS bank count assumed by the photon scheduler (32 or 64; set before frames)
+ *   3 = LDS bank model of the photon scheduler (32 = conflict-free half-waves, the
+ *       default and gfx950's layout; 64; 0 = no scheduling; set before frames)
  *   4 = hypothesis segments per batch (default automatic)
- *   5 = per-kernel timing with HIP events (1 on; read with rsp_emc_timing)
- *   6 = cap on the float4 per thread of the next table prefetched into registers
- *       (-1 = as many as the instantiation holds, 0 = no prefetch) */
+ *   5 = per-kernel timing with HIP events (1 on; read with rsp_emc_timing) */
 int rsp_emc_set_option(rsp_emc* plan, int key, int64_t value);
 
 /* Run the E-step for `model_c` (device, npix complex) with per-frame `scales` (device,

@@ -34,7 +34,14 @@
 namespace rsp {
 
 constexpr int kNPMax = 4;        // table parts (LDS-sized pieces of the scored annulus)
-constexpr int kPartMax = 38400;  // floats per part: 150 KB of the 160 KB LDS
+// The scored annulus is cut into parts of kPartMax entries; a part buffer in LDS holds
+// kChunks 1 KB chunks (kPStride floats: the entries, then >= 32 zero pad entries), and a
+// workgroup keeps kNBuf part buffers (150.5 KB of the 160 KB LDS) as a ring fed by
+// asynchronous direct-to-LDS loads two steps ahead of the gather.
+constexpr int kChunks = 49;
+constexpr int kPStride = kChunks * 256;
+constexpr int kPartMax = kPStride - 32;
+constexpr int kNBuf = 3;
 
 struct Top3 {
   double v1, v2, v3;
@@ -143,7 +150,7 @@ __global__ __launch_bounds__(256) void k_tables(
     const int32_t* __restrict__ scored_t, int n_scored, int size, int R,
     const double* __restrict__ rcos, const double* __restrict__ rsin, int hb, int nb,
     int HC, int PSTRIDE, int TS, float* __restrict__ tables,
-    unsigned* __restrict__ errmax) {
+    unsigned* __restrict__ errmax, unsigned* __restrict__ absmax) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n_scored) return;
   const int t = scored_t[i];
@@ -152,55 +159,41 @@ __global__ __launch_bounds__(256) void k_tables(
   const int64_t col = (int64_t)part * PSTRIDE + (i - part * kPartMax);
   const int h0 = hb + blockIdx.y * HC;
   const int h1 = min(hb + nb, h0 + HC);
-  float emax = 0.f;
+  float emax = 0.f, amax = 0.f;
   for (int h = h0; h < h1; ++h) {
     const int s = h / R, r = h - s * R;
     const double lv = logview_exact(I + (int64_t)s * npix, bg, size, t, rcos[r], rsin[r]);
     const float f = (float)lv;
     tables[(int64_t)(h - hb) * TS + col] = f;
-    emax = fmaxf(emax, (float)fabs((double)f - lv));
+    // the rounding error, rounded up to float
+    const double d = fabs((double)f - lv);
+    float df = (float)d;
+    if ((double)df < d) df = nextafterf(df, INFINITY);
+    emax = fmaxf(emax, df);
+    amax = fmaxf(amax, fabsf(f));
   }
   // max of non-negative floats == max of their bit patterns: order independent
   atomicMax(errmax + i, __float_as_uint(emax));
+  atomicMax(absmax + i, __float_as_uint(amax));
 }
 
-// Compile-time walk over the slot registers in blocks of 16 so that every a[k] is a
-// statically named VGPR (a runtime-indexed loop would leave the array in scratch).  The
-// 16 LDS reads of a block are issued before the 16 FP64 additions that consume them.
+// Compile-time walk over the slot registers in blocks of 8 so that every a[k] is a
+// statically named VGPR (a runtime-indexed loop would leave the array in scratch).  Each
+// block is summed pairwise in FP32 (error <= 3 * 2^-24 * sum|v|, accounted for in the
+// per-frame bound of k_resolve) and added to an FP64 accumulator.
 template <int K0, int P>
 __device__ __forceinline__ void gather_blocks(const char* lds, const uint32_t (&a)[P], int kbeg,
                                               int kend, double (&acc)[4]) {
   if constexpr (K0 < P) {
     if (K0 >= kend) return;
     if (K0 >= kbeg) {
-      float v[16];
+      float v[8];
 #pragma unroll
-      for (int u = 0; u < 16; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K0 + u]);
-#pragma unroll
-      for (int u = 0; u < 16; ++u) acc[u & 3] += (double)v[u];
+      for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K0 + u]);
+      const float s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+      acc[(K0 >> 3) & 3] += (double)s8;
     }
-    gather_blocks<K0 + 16, P>(lds, a, kbeg, kend, acc);
-  }
-}
-
-// native 4-float vector (HIP's float4 struct is copied through memcpy, which keeps a
-// register array of it in scratch)
-typedef float f4v __attribute__((ext_vector_type(4)));
-
-// prefetch buffer helpers (compile-time indices keep pf[] in VGPRs)
-template <int U, int PF, int T>
-__device__ __forceinline__ void pf_load(f4v (&pf)[PF], const f4v* __restrict__ src, int npf) {
-  if constexpr (U < PF) {
-    if (U < npf) pf[U] = src[threadIdx.x + U * T];
-    pf_load<U + 1, PF, T>(pf, src, npf);
-  }
-}
-
-template <int U, int PF, int T>
-__device__ __forceinline__ void pf_store(const f4v (&pf)[PF], f4v* lds4, int npf) {
-  if constexpr (U < PF) {
-    if (U < npf) lds4[threadIdx.x + U * T] = pf[U];
-    pf_store<U + 1, PF, T>(pf, lds4, npf);
+    gather_blocks<K0 + 8, P>(lds, a, kbeg, kend, acc);
   }
 }
 
@@ -218,16 +211,65 @@ __device__ __forceinline__ void load_slots(uint32_t (&a)[P], const uint32_t* __r
 // address (within the table part resident in LDS) of the frame's k-th photon entry;
 // wave_meta[gw*kNPMax + p] is the end slot of part p (a multiple of 8).
 // --------------------------------------------------------------------------------
-template <int P, int NW, int PF>
+// One (hypothesis, part) step of the gather: wait until this step's part is in LDS
+// (this wave's loads via vmcnt, every wave's via the barrier), start the loads of the
+// step two ahead into the buffer freed by the previous step, then gather.
+template <int P, int NW, int L>
+struct GatherStep {
+  const float* tables;
+  int TS, NP, hb, h0, nq, R, wid, lane;
+  double scale;
+
+  // Loads of step q into dst.  Steps past the end re-load the first part (in bounds) into
+  // a buffer that is never read again: every step issues exactly L loads, so the wait
+  // counts are the same on every path and the compiler adds no waits of its own.
+  __device__ __forceinline__ void issue(int q, float* dst) const {
+    if (q >= nq) q = 0;
+    const int h = h0 + q / NP, p = q - (q / NP) * NP;
+    const float* src = tables + (int64_t)(h - hb) * TS + (int64_t)p * kPStride;
+#pragma unroll
+    for (int u = 0; u < L; ++u) {
+      const int c = min(wid + u * NW, kChunks - 1);
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void*)(src + (c * 64 + lane) * 4),
+          (__attribute__((address_space(3))) void*)(dst + c * 256), 16, 0, 0);
+    }
+  }
+
+  __device__ __forceinline__ void run(int q, const float* cur, float* ahead, const int4 pend,
+                                      const uint32_t (&a)[P], const double* __restrict__ init,
+                                      double (&acc)[4], int& kbeg, Top3& top) const {
+    const int h = h0 + q / NP, p = q - (q / NP) * NP;
+    // s_waitcnt through the builtin so that the compiler's own wait bookkeeping sees it
+    // (gfx9 encoding: vmcnt[3:0], expcnt[6:4] = 7, lgkmcnt[11:8] = 15: no wait on those)
+    static_assert(L < 16, "vmcnt immediate");
+    __builtin_amdgcn_s_waitcnt(0xF70 | L);
+    // a bare barrier: a fenced __syncthreads would also wait for the loads of step q+1
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    issue(q + 2, ahead);
+    if (p == 0) {
+      acc[0] = init[h / R] * scale;
+      acc[1] = acc[2] = acc[3] = 0.;
+      kbeg = 0;
+    }
+    const int kend = p == 0 ? pend.x : p == 1 ? pend.y : p == 2 ? pend.z : pend.w;
+    gather_blocks<0, P>(reinterpret_cast<const char*>(cur), a, kbeg, kend, acc);
+    kbeg = kend;
+    if (p == NP - 1) top3_insert(top, (acc[0] + acc[1]) + (acc[2] + acc[3]), h);
+  }
+};
+
+template <int P, int NW, int L>
 __global__ __launch_bounds__(64 * NW) void k_gather(
-    const float* __restrict__ tables, int TS, int PSTRIDE, int NP, int hb, int nb,
-    int nseg, int n_tiles, int gw0, const uint32_t* __restrict__ slots,
-    const int32_t* __restrict__ wave_meta, const double* __restrict__ scale_sorted,
-    const double* __restrict__ init, int R, Top3* __restrict__ partials, int Npad,
-    int pf_cap) {
-  extern __shared__ f4v lds4[];
-  const char* lds = reinterpret_cast<const char*>(lds4);
-  constexpr int T = 64 * NW;
+    const float* __restrict__ tables, int TS, int NP, int hb, int nb, int nseg, int n_tiles,
+    int gw0, const uint32_t* __restrict__ slots, const int32_t* __restrict__ wave_meta,
+    const double* __restrict__ scale_sorted, const double* __restrict__ init, int R,
+    Top3* __restrict__ partials, int Npad) {
+  static_assert(L * NW >= kChunks, "L loads per wave must cover the part");
+  // three separate LDS objects (not one array indexed by q % 3): the compiler then knows
+  // which buffer each direct-to-LDS load writes and waits only for the buffer it reads
+  __shared__ float buf0[kPStride], buf1[kPStride], buf2[kPStride];
   const int b = blockIdx.x;
   int seg, tile;
   if ((nseg & 7) == 0) {  // keep one segment's workgroups on one XCD (round-robin dispatch)
@@ -247,47 +289,32 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
 
   uint32_t a[P];
   load_slots<0, P>(a, slots + (int64_t)lw * P * 64 + lane);
-  int pend[kNPMax];
-#pragma unroll
-  for (int p = 0; p < kNPMax; ++p) pend[p] = wave_meta[gw * kNPMax + p];
-  const double scale = scale_sorted[j];
+  static_assert(kNPMax == 4, "part ends are held in an int4");
+  // end slot of each part (wave-uniform)
+  const int4 pend = make_int4(__builtin_amdgcn_readfirstlane(wave_meta[gw * kNPMax + 0]),
+                              __builtin_amdgcn_readfirstlane(wave_meta[gw * kNPMax + 1]),
+                              __builtin_amdgcn_readfirstlane(wave_meta[gw * kNPMax + 2]),
+                              __builtin_amdgcn_readfirstlane(wave_meta[gw * kNPMax + 3]));
   Top3 top = top3_empty();
-  const int n4 = PSTRIDE >> 2;
-  // the first npf*T float4 of each table part are prefetched into registers one step
-  // ahead (overlapping the gather); the short tail is copied synchronously
-  const int npf = min(min(PF, n4 / T), pf_cap);
-  const int tail0 = npf * T;
-  const int nq = (h1 - h0) * NP;  // steps over (hypothesis, part)
-  f4v pf[PF];
-  auto src_of = [&](int q) {
-    const int h = h0 + q / NP, p = q - (q / NP) * NP;
-    return reinterpret_cast<const f4v*>(tables + (int64_t)(h - hb) * TS + (int64_t)p * PSTRIDE);
-  };
-  if (nq > 0) pf_load<0, PF, T>(pf, src_of(0), npf);
+  // step q: part q % NP of hypothesis h0 + q / NP, in buffer q % 3.  Every wave issues
+  // exactly L 1 KB direct-to-LDS loads per step (surplus loads repeat the last chunk with
+  // identical bytes), so "vmcnt <= L" means this wave's loads of the older step landed.
+  const GatherStep<P, NW, L> st{tables, TS, NP, hb, h0, (h1 - h0) * NP, R, wid, lane,
+                                scale_sorted[j]};
+  st.issue(0, buf0);
+  st.issue(1, buf1);
   double acc[4];
   int kbeg = 0;
-  for (int q = 0; q < nq; ++q) {
-    const int h = h0 + q / NP, p = q - (q / NP) * NP;
-    if (p == 0) {
-      acc[0] = init[h / R] * scale;
-      acc[1] = acc[2] = acc[3] = 0.;
-      kbeg = 0;
-    }
-    const f4v* src = src_of(q);
-    __syncthreads();
-    pf_store<0, PF, T>(pf, lds4, npf);
-#pragma unroll 4
-    for (int i = tail0 + threadIdx.x; i < n4; i += T) lds4[i] = src[i];
-    __syncthreads();
-    if (q + 1 < nq) pf_load<0, PF, T>(pf, src_of(q + 1), npf);
-    int kend = pend[0];
-#pragma unroll
-    for (int pp = 1; pp < kNPMax; ++pp)
-      if (p == pp) kend = pend[pp];
-    gather_blocks<0, P>(lds, a, kbeg, kend, acc);
-    kbeg = kend;
-    if (p == NP - 1) top3_insert(top, (acc[0] + acc[1]) + (acc[2] + acc[3]), h);
+  int q = 0;
+  for (; q + 3 <= st.nq; q += 3) {
+    st.run(q, buf0, buf2, pend, a, init, acc, kbeg, top);
+    st.run(q + 1, buf1, buf0, pend, a, init, acc, kbeg, top);
+    st.run(q + 2, buf2, buf1, pend, a, init, acc, kbeg, top);
   }
+  if (q < st.nq) st.run(q, buf0, buf2, pend, a, init, acc, kbeg, top);
+  if (q + 1 < st.nq) st.run(q + 1, buf1, buf0, pend, a, init, acc, kbeg, top);
+  // no direct-to-LDS write may still be in flight when the workgroup's LDS is released
+  __builtin_amdgcn_s_waitcnt(0xF70);
   partials[(int64_t)seg * Npad + j] = top;
 }
 
@@ -354,6 +381,7 @@ __global__ __launch_bounds__(256) void k_resolve(ExactArgs A, const Top3* __rest
                                                  const int32_t* __restrict__ perm, int Npad,
                                                  const int32_t* __restrict__ compact_of,
                                                  const unsigned* __restrict__ errmax,
+                                                 const unsigned* __restrict__ absmax,
                                                  int exact_mode, int32_t* __restrict__ rmax,
                                                  double* __restrict__ maxll,
                                                  int32_t* __restrict__ full_list,
@@ -371,7 +399,10 @@ __global__ __launch_bounds__(256) void k_resolve(ExactArgs A, const Top3* __rest
   double E = 0.;
   for (int64_t e = A.ex_off[d]; e < A.ex_off[d + 1]; ++e) {
     const int c = (e < A.ex_off[d] + A.ex_n1[d]) ? 1 : A.ex_cnt[e];
-    E += (double)c * (double)__uint_as_float(errmax[compact_of[A.ex_pix[e]]]);
+    const int ci = compact_of[A.ex_pix[e]];
+    // table rounding + the FP32 pairwise 8-sums of the gather (gamma_3 * |value|)
+    E += (double)c * ((double)__uint_as_float(errmax[ci]) +
+                      1.8e-7 * (double)__uint_as_float(absmax[ci]));
   }
   // plus the FP64 summation error of both paths: n*2^-53*sum|terms| with |log-view| <= 100
   // (values of -1000 mark out-of-range samples and never win); 1000x safety factor
@@ -486,6 +517,7 @@ struct rsp_emc {
   int32_t *d_probmask = nullptr, *d_scored_t = nullptr, *d_compact_of = nullptr;
   float* d_tables = nullptr;
   unsigned* d_errmax = nullptr;
+  unsigned* d_absmax = nullptr;
   int64_t batch = 0;
   // frames
   int64_t N = 0, Npad = 0;
@@ -506,7 +538,6 @@ struct rsp_emc {
   // options / stats
   int exact_mode = 0;
   int64_t opt_batch = 0, opt_nseg = 0;
-  int pf_cap = 1 << 20;  // max float4 per thread prefetched into registers
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   struct Class {
     int cfg, n_tiles, gw0;
@@ -545,7 +576,7 @@ int dev_upload(T** p, const std::vector<T>& v) {
 void free_all(rsp_emc* e) {
   void* ptrs[] = {e->d_rcos, e->d_rsin, e->d_sx, e->d_sy, e->d_dia, e->d_bg, e->d_I,
                   e->d_init, e->d_vscale, e->d_probmask, e->d_scored_t, e->d_compact_of,
-                  e->d_tables, e->d_errmax, e->d_slots, e->d_meta, e->d_perm,
+                  e->d_tables, e->d_errmax, e->d_absmax, e->d_slots, e->d_meta, e->d_perm,
                   e->d_scale_sorted, e->d_partials, e->d_running, e->d_ex_off, e->d_ex_n1,
                   e->d_ex_pix, e->d_ex_cnt, e->d_full_list, e->d_counters, e->d_bval,
                   e->d_bidx};
@@ -600,24 +631,23 @@ int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
   return nslots;
 }
 
-using GatherFn = void (*)(const float*, int, int, int, int, int, int, int, int,
-                          const uint32_t*, const int32_t*, const double*, const double*, int,
-                          Top3*, int, int);
+using GatherFn = void (*)(const float*, int, int, int, int, int, int, int, const uint32_t*,
+                          const int32_t*, const double*, const double*, int, Top3*, int);
 
 // Gather instantiations: P slot registers per lane, NW waves per workgroup (so that the
-// P slots + ~40 working VGPRs + 4*PF prefetch VGPRs fit NW/4 waves per SIMD), PF float4
-// of the next table prefetched per thread.
+// P slots + ~45 working VGPRs fit NW/4 waves per SIMD), L = ceil(kChunks / NW) direct-to-
+// LDS loads per wave per table part.
 struct GatherCfg {
-  int P, NW, PF;
+  int P, NW;
   GatherFn fn;
 };
 
 const GatherCfg kGatherCfgs[] = {
-    {16, 16, 8, &k_gather<16, 16, 8>},   {32, 16, 8, &k_gather<32, 16, 8>},
-    {48, 16, 7, &k_gather<48, 16, 7>},   {64, 16, 3, &k_gather<64, 16, 3>},
-    {80, 12, 7, &k_gather<80, 12, 7>},   {96, 12, 5, &k_gather<96, 12, 5>},
-    {128, 8, 12, &k_gather<128, 8, 12>}, {160, 8, 10, &k_gather<160, 8, 10>},
-    {192, 8, 3, &k_gather<192, 8, 3>},
+    {16, 16, &k_gather<16, 16, 4>},  {32, 16, &k_gather<32, 16, 4>},
+    {48, 16, &k_gather<48, 16, 4>},  {64, 16, &k_gather<64, 16, 4>},
+    {80, 12, &k_gather<80, 12, 5>},  {96, 12, &k_gather<96, 12, 5>},
+    {112, 8, &k_gather<112, 8, 7>},  {128, 8, &k_gather<128, 8, 7>},
+    {160, 8, &k_gather<160, 8, 7>},
 };
 constexpr int kNumCfg = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
 
@@ -677,8 +707,7 @@ extern "C" int rsp_emc_create(int device, int64_t size, int64_t num_rot, const d
     delete e;
     return fail(kErrUnsupported, "rsp_emc_create: scored annulus too large for LDS parts");
   }
-  const int part_len = std::min(e->n_scored, kPartMax);
-  e->PSTRIDE = (int)(ceil_div(part_len + 64, 4) * 4);  // + pad entries (zeros), 16 B rows
+  e->PSTRIDE = kPStride;
   e->TS = e->PSTRIDE * e->NP;
   if ((rc = dev_upload(&e->d_rcos, e->rcos)) || (rc = dev_upload(&e->d_rsin, e->rsin)) ||
       (rc = dev_upload(&e->d_sx, e->ssx)) || (rc = dev_upload(&e->d_sy, e->ssy)) ||
@@ -687,7 +716,7 @@ extern "C" int rsp_emc_create(int device, int64_t size, int64_t num_rot, const d
       (rc = dev_upload(&e->d_compact_of, e->compact_of)) ||
       (rc = dev_alloc(&e->d_I, (int64_t)e->S * e->npix)) ||
       (rc = dev_alloc(&e->d_init, e->S)) || (rc = dev_alloc(&e->d_vscale, 1)) ||
-      (rc = dev_alloc(&e->d_errmax, e->n_scored)) || (rc = dev_alloc(&e->d_counters, 4))) {
+      (rc = dev_alloc(&e->d_errmax, e->n_scored)) || (rc = dev_alloc(&e->d_absmax, e->n_scored)) || (rc = dev_alloc(&e->d_counters, 4))) {
     free_all(e);
     delete e;
     return rc;
@@ -714,7 +743,6 @@ extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
       e->opt_nseg = value;
       return kOk;
     case 5: e->timing = value != 0; return kOk;
-    case 6: e->pf_cap = value < 0 ? (1 << 20) : (int)value; return kOk;
     default: return fail(kErrSize, "rsp_emc_set_option: unknown key");
   }
 }
@@ -826,7 +854,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
           for (size_t r = 0; r < ent[l].size(); ++r) pl[slot_of[l][r]] = ent[l][r];
         }
       }
-      part_slots = (int)ceil_div(part_slots, 16) * 16;
+      part_slots = (int)ceil_div(part_slots, 8) * 8;
       const int part_len = std::min(kPartMax, e->n_scored - p * kPartMax);
       for (int g0 = 0; g0 < 64; g0 += G) {
         for (int k = 0; k < part_slots; ++k) {
@@ -840,7 +868,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
             if ((int)pl.size() <= k) pl.resize(k + 1, -1);
             if (pl[k] >= 0) continue;
             if (NB == 0) {
-              pl[k] = part_len + l;  // distinct zero pads
+              pl[k] = part_len + (l % 32);  // zero pads
               continue;
             }
             int bank = 0;
@@ -866,7 +894,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   std::vector<int> wcfg(nw0);
   for (int64_t w = nw0 - 1; w >= 0; --w) {
     int c = choose_cfg(std::max(wave_pend[w][kNPMax - 1], 16));
-    if (c < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs more than 192 slots");
+    if (c < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs more than 160 slots");
     if (w + 1 < nw0) c = std::max(c, wcfg[w + 1]);
     wcfg[w] = c;
   }
@@ -907,7 +935,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
           if (real && k < (int)wave_slots[w][l].size())
             entry = wave_slots[w][l][k];
           else
-            entry = (int)never_read + (l % 64);  // a zero pad (never read if k >= end)
+            entry = (int)never_read + (l % 32);  // a zero pad (never read if k >= end)
           slots[(size_t)(c.slot_off + ((int64_t)lw * g.P + k) * 64 + l)] = (uint32_t)entry * 4u;
         }
       }
@@ -974,11 +1002,9 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
                      scales, e->d_perm, (int)e->Npad, e->d_scale_sorted);
   RSP_LAUNCH_CHECK("k_scatter_scale");
   RSP_HIP(hipMemsetAsync(e->d_errmax, 0, sizeof(unsigned) * e->n_scored, st));
+  RSP_HIP(hipMemsetAsync(e->d_absmax, 0, sizeof(unsigned) * e->n_scored, st));
   RSP_HIP(hipMemsetAsync(e->d_counters, 0, sizeof(unsigned) * 4, st));
-  const size_t lds_bytes = sizeof(float) * (size_t)e->PSTRIDE;
-  for (const auto& c : e->classes)
-    RSP_HIP(hipFuncSetAttribute((const void*)kGatherCfgs[c.cfg].fn,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
+  const size_t lds_bytes = 0;  // the gather's three part buffers are static LDS
   if (!e->exact_mode) {
     const int HC = 64;
     int first = 1;
@@ -999,7 +1025,7 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 0], st));
       hipLaunchKernelGGL(k_tables, tg, dim3(256), 0, st, e->d_I, e->d_bg, e->d_scored_t,
                          e->n_scored, e->size, e->R, e->d_rcos, e->d_rsin, (int)hb, nb, HC,
-                         e->PSTRIDE, e->TS, e->d_tables, e->d_errmax);
+                         e->PSTRIDE, e->TS, e->d_tables, e->d_errmax, e->d_absmax);
       RSP_LAUNCH_CHECK("k_tables");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 1], st));
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 2], st));
@@ -1010,9 +1036,9 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
         sm.gw_end[ci] = c.gw0 + c.n_tiles * gc.NW;
         sm.nseg[ci] = ns;
         hipLaunchKernelGGL(gc.fn, dim3((unsigned)(ns * c.n_tiles)), dim3(64 * gc.NW), lds_bytes,
-                           st, e->d_tables, e->TS, e->PSTRIDE, e->NP, (int)hb, nb, ns, c.n_tiles,
-                           c.gw0, e->d_slots + c.slot_off, e->d_meta, e->d_scale_sorted,
-                           e->d_init, e->R, e->d_partials, (int)e->Npad, e->pf_cap);
+                           st, e->d_tables, e->TS, e->NP, (int)hb, nb, ns, c.n_tiles, c.gw0,
+                           e->d_slots + c.slot_off, e->d_meta, e->d_scale_sorted, e->d_init,
+                           e->R, e->d_partials, (int)e->Npad);
       }
       RSP_LAUNCH_CHECK("k_gather");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));
@@ -1026,7 +1052,7 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
               e->d_ex_n1, e->d_ex_pix, e->d_ex_cnt, e->size, e->R};
   hipLaunchKernelGGL(k_resolve, dim3((unsigned)ceil_div(e->Npad, 256)), dim3(256), 0, st, A,
                      e->d_running, e->d_perm, (int)e->Npad, e->d_compact_of, e->d_errmax,
-                     e->exact_mode, rmax, maxll, e->d_full_list, e->d_counters);
+                     e->d_absmax, e->exact_mode, rmax, maxll, e->d_full_list, e->d_counters);
   RSP_LAUNCH_CHECK("k_resolve");
   unsigned cnt[4] = {0, 0, 0, 0};
   RSP_HIP(hipMemcpyAsync(cnt, e->d_counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
