@@ -177,23 +177,21 @@ __global__ __launch_bounds__(256) void k_tables(
   atomicMax(absmax + i, __float_as_uint(amax));
 }
 
-// Compile-time walk over the slot registers in blocks of 8 so that every a[k] is a
-// statically named VGPR (a runtime-indexed loop would leave the array in scratch).  Each
-// block is summed pairwise in FP32 (error <= 3 * 2^-24 * sum|v|, accounted for in the
-// per-frame bound of k_resolve) and added to an FP64 accumulator.
-template <int K0, int P>
-__device__ __forceinline__ void gather_blocks(const char* lds, const uint32_t (&a)[P], int kbeg,
-                                              int kend, double (&acc)[4]) {
-  if constexpr (K0 < P) {
-    if (K0 >= kend) return;
-    if (K0 >= kbeg) {
-      float v[8];
+// Straight-line walk over the slot registers [K0, K1) in blocks of 8: every a[k] is a
+// statically named VGPR and, with no branches between blocks, the LDS reads of one block
+// overlap the arithmetic of the previous one.  Each block is summed pairwise in FP32
+// (error <= 3 * 2^-24 * sum|v|, accounted for in the per-frame bound of k_resolve) and
+// added to an FP64 accumulator.
+template <int K0, int K1, int P>
+__device__ __forceinline__ void gather_range(const char* lds, const uint32_t (&a)[P],
+                                             double (&acc)[4]) {
+  if constexpr (K0 < K1) {
+    float v[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K0 + u]);
-      const float s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
-      acc[(K0 >> 3) & 3] += (double)s8;
-    }
-    gather_blocks<K0 + 8, P>(lds, a, kbeg, kend, acc);
+    for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K0 + u]);
+    const float s8 = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+    acc[(K0 >> 3) & 3] += (double)s8;
+    gather_range<K0 + 8, K1, P>(lds, a, acc);
   }
 }
 
@@ -211,18 +209,32 @@ __device__ __forceinline__ void load_slots(uint32_t (&a)[P], const uint32_t* __r
 // address (within the table part resident in LDS) of the frame's k-th photon entry;
 // wave_meta[gw*kNPMax + p] is the end slot of part p (a multiple of 8).
 // --------------------------------------------------------------------------------
-// One (hypothesis, part) step of the gather: wait until this step's part is in LDS
-// (this wave's loads via vmcnt, every wave's via the barrier), start the loads of the
-// step two ahead into the buffer freed by the previous step, then gather.
-template <int P, int NW, int L>
+// The three LDS part buffers of the gather, as separate objects: the compiler then knows
+// which buffer each direct-to-LDS load writes and waits only for the buffer it reads.
+__shared__ float g_buf0[kPStride], g_buf1[kPStride], g_buf2[kPStride];
+
+template <int B>
+__device__ __forceinline__ float* part_buf() {
+  if constexpr (B == 0) return g_buf0;
+  else if constexpr (B == 1) return g_buf1;
+  else return g_buf2;
+}
+
+// The gather walks steps q = (hypothesis, part) in groups of 3 hypotheses so that the
+// part index and the LDS buffer (q % 3) of every step are compile-time constants.  Step
+// q: wait until its part is in LDS (this wave's loads via vmcnt, every wave's via the
+// barrier), start the loads of step q+2 into the buffer freed by step q-1, then gather the
+// part's S slots.  Template: S slots per part and lane, NP parts, NW waves, L loads/wave.
+template <int S, int NP, int NW, int L>
 struct GatherStep {
+  static constexpr int P = S * NP;
   const float* tables;
-  int TS, NP, hb, h0, nq, R, wid, lane;
+  int TS, hb, h0, h1, nq, R, wid, lane;
   double scale;
 
   // Loads of step q into dst.  Steps past the end re-load the first part (in bounds) into
-  // a buffer that is never read again: every step issues exactly L loads, so the wait
-  // counts are the same on every path and the compiler adds no waits of its own.
+  // a buffer that is never read for a real hypothesis: every step issues exactly L loads,
+  // so the wait counts are the same on every path and the compiler adds no waits.
   __device__ __forceinline__ void issue(int q, float* dst) const {
     if (q >= nq) q = 0;
     const int h = h0 + q / NP, p = q - (q / NP) * NP;
@@ -236,10 +248,14 @@ struct GatherStep {
     }
   }
 
-  __device__ __forceinline__ void run(int q, const float* cur, float* ahead, const int4 pend,
-                                      const uint32_t (&a)[P], const double* __restrict__ init,
-                                      double (&acc)[4], int& kbeg, Top3& top) const {
-    const int h = h0 + q / NP, p = q - (q / NP) * NP;
+  // static step QS (0 .. 3*NP-1) of group g
+  template <int QS>
+  __device__ __forceinline__ void step(int g, const uint32_t (&a)[P],
+                                       const double* __restrict__ init, double (&acc)[4],
+                                       Top3& top) const {
+    constexpr int PP = QS % NP;
+    const int q = g * 3 * NP + QS;
+    const int h = h0 + g * 3 + QS / NP;
     // s_waitcnt through the builtin so that the compiler's own wait bookkeeping sees it
     // (gfx9 encoding: vmcnt[3:0], expcnt[6:4] = 7, lgkmcnt[11:8] = 15: no wait on those)
     static_assert(L < 16, "vmcnt immediate");
@@ -247,29 +263,36 @@ struct GatherStep {
     // a bare barrier: a fenced __syncthreads would also wait for the loads of step q+1
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    issue(q + 2, ahead);
-    if (p == 0) {
-      acc[0] = init[h / R] * scale;
+    issue(q + 2, part_buf<(QS + 2) % 3>());
+    if constexpr (PP == 0) {
+      acc[0] = init[min(h, h1 - 1) / R] * scale;
       acc[1] = acc[2] = acc[3] = 0.;
-      kbeg = 0;
     }
-    const int kend = p == 0 ? pend.x : p == 1 ? pend.y : p == 2 ? pend.z : pend.w;
-    gather_blocks<0, P>(reinterpret_cast<const char*>(cur), a, kbeg, kend, acc);
-    kbeg = kend;
-    if (p == NP - 1) top3_insert(top, (acc[0] + acc[1]) + (acc[2] + acc[3]), h);
+    gather_range<PP * S, PP * S + S, P>(reinterpret_cast<const char*>(part_buf<QS % 3>()), a,
+                                        acc);
+    if constexpr (PP == NP - 1) {
+      if (h < h1) top3_insert(top, (acc[0] + acc[1]) + (acc[2] + acc[3]), h);
+    }
+  }
+
+  template <int QS>
+  __device__ __forceinline__ void group(int g, const uint32_t (&a)[P],
+                                        const double* __restrict__ init, double (&acc)[4],
+                                        Top3& top) const {
+    if constexpr (QS < 3 * NP) {
+      step<QS>(g, a, init, acc, top);
+      group<QS + 1>(g, a, init, acc, top);
+    }
   }
 };
 
-template <int P, int NW, int L>
+template <int S, int NP, int NW, int L>
 __global__ __launch_bounds__(64 * NW) void k_gather(
-    const float* __restrict__ tables, int TS, int NP, int hb, int nb, int nseg, int n_tiles,
-    int gw0, const uint32_t* __restrict__ slots, const int32_t* __restrict__ wave_meta,
-    const double* __restrict__ scale_sorted, const double* __restrict__ init, int R,
-    Top3* __restrict__ partials, int Npad) {
+    const float* __restrict__ tables, int TS, int hb, int nb, int nseg, int n_tiles, int gw0,
+    const uint32_t* __restrict__ slots, const double* __restrict__ scale_sorted,
+    const double* __restrict__ init, int R, Top3* __restrict__ partials, int Npad) {
   static_assert(L * NW >= kChunks, "L loads per wave must cover the part");
-  // three separate LDS objects (not one array indexed by q % 3): the compiler then knows
-  // which buffer each direct-to-LDS load writes and waits only for the buffer it reads
-  __shared__ float buf0[kPStride], buf1[kPStride], buf2[kPStride];
+  constexpr int P = S * NP;
   const int b = blockIdx.x;
   int seg, tile;
   if ((nseg & 7) == 0) {  // keep one segment's workgroups on one XCD (round-robin dispatch)
@@ -289,30 +312,17 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
 
   uint32_t a[P];
   load_slots<0, P>(a, slots + (int64_t)lw * P * 64 + lane);
-  static_assert(kNPMax == 4, "part ends are held in an int4");
-  // end slot of each part (wave-uniform)
-  const int4 pend = make_int4(__builtin_amdgcn_readfirstlane(wave_meta[gw * kNPMax + 0]),
-                              __builtin_amdgcn_readfirstlane(wave_meta[gw * kNPMax + 1]),
-                              __builtin_amdgcn_readfirstlane(wave_meta[gw * kNPMax + 2]),
-                              __builtin_amdgcn_readfirstlane(wave_meta[gw * kNPMax + 3]));
   Top3 top = top3_empty();
-  // step q: part q % NP of hypothesis h0 + q / NP, in buffer q % 3.  Every wave issues
-  // exactly L 1 KB direct-to-LDS loads per step (surplus loads repeat the last chunk with
-  // identical bytes), so "vmcnt <= L" means this wave's loads of the older step landed.
-  const GatherStep<P, NW, L> st{tables, TS, NP, hb, h0, (h1 - h0) * NP, R, wid, lane,
-                                scale_sorted[j]};
-  st.issue(0, buf0);
-  st.issue(1, buf1);
+  // every wave issues exactly L 1 KB direct-to-LDS loads per step (surplus loads repeat the
+  // last chunk with identical bytes), so "vmcnt <= L" means: this wave's loads of the
+  // older step have landed
+  const GatherStep<S, NP, NW, L> st{tables, TS, hb, h0, h1, (h1 - h0) * NP, R, wid, lane,
+                                    scale_sorted[j]};
+  st.issue(0, g_buf0);
+  st.issue(1, g_buf1);
   double acc[4];
-  int kbeg = 0;
-  int q = 0;
-  for (; q + 3 <= st.nq; q += 3) {
-    st.run(q, buf0, buf2, pend, a, init, acc, kbeg, top);
-    st.run(q + 1, buf1, buf0, pend, a, init, acc, kbeg, top);
-    st.run(q + 2, buf2, buf1, pend, a, init, acc, kbeg, top);
-  }
-  if (q < st.nq) st.run(q, buf0, buf2, pend, a, init, acc, kbeg, top);
-  if (q + 1 < st.nq) st.run(q + 1, buf1, buf0, pend, a, init, acc, kbeg, top);
+  const int ngroups = (h1 - h0 + 2) / 3;
+  for (int g = 0; g < ngroups; ++g) st.template group<0>(g, a, init, acc, top);
   // no direct-to-LDS write may still be in flight when the workgroup's LDS is released
   __builtin_amdgcn_s_waitcnt(0xF70);
   partials[(int64_t)seg * Npad + j] = top;
@@ -631,29 +641,41 @@ int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
   return nslots;
 }
 
-using GatherFn = void (*)(const float*, int, int, int, int, int, int, int, const uint32_t*,
-                          const int32_t*, const double*, const double*, int, Top3*, int);
+using GatherFn = void (*)(const float*, int, int, int, int, int, int, const uint32_t*,
+                          const double*, const double*, int, Top3*, int);
 
-// Gather instantiations: P slot registers per lane, NW waves per workgroup (so that the
-// P slots + ~45 working VGPRs fit NW/4 waves per SIMD), L = ceil(kChunks / NW) direct-to-
-// LDS loads per wave per table part.
+// Gather instantiations: S slot registers per table part and lane, NP parts, NW waves per
+// workgroup so that the S*NP slots + ~50 working VGPRs fit NW/4 waves per SIMD, and
+// L = ceil(kChunks / NW) direct-to-LDS loads per wave per part.
+constexpr int nw_of(int P, int NP) {
+  return NP == 1 ? (P <= 64 ? 16 : P <= 96 ? 12 : 8)
+                 : NP == 2 ? (P <= 64 ? 16 : P <= 80 ? 12 : 8) : (P <= 48 ? 16 : 8);
+}
+constexpr int l_of(int P, int NP) { return (kChunks + nw_of(P, NP) - 1) / nw_of(P, NP); }
+
 struct GatherCfg {
-  int P, NW;
+  int S, NP, NW;
   GatherFn fn;
 };
 
+#define RSP_CFG(S, NP) \
+  {S, NP, nw_of(S * NP, NP), &k_gather<S, NP, nw_of(S * NP, NP), l_of(S * NP, NP)>}
 const GatherCfg kGatherCfgs[] = {
-    {16, 16, &k_gather<16, 16, 4>},  {32, 16, &k_gather<32, 16, 4>},
-    {48, 16, &k_gather<48, 16, 4>},  {64, 16, &k_gather<64, 16, 4>},
-    {80, 12, &k_gather<80, 12, 5>},  {96, 12, &k_gather<96, 12, 5>},
-    {112, 8, &k_gather<112, 8, 7>},  {128, 8, &k_gather<128, 8, 7>},
-    {160, 8, &k_gather<160, 8, 7>},
+    RSP_CFG(16, 1), RSP_CFG(32, 1), RSP_CFG(48, 1), RSP_CFG(64, 1), RSP_CFG(80, 1),
+    RSP_CFG(96, 1), RSP_CFG(112, 1), RSP_CFG(128, 1), RSP_CFG(160, 1),
+    RSP_CFG(8, 2),  RSP_CFG(16, 2), RSP_CFG(24, 2), RSP_CFG(32, 2), RSP_CFG(40, 2),
+    RSP_CFG(48, 2), RSP_CFG(56, 2), RSP_CFG(64, 2), RSP_CFG(80, 2),
+    RSP_CFG(8, 3),  RSP_CFG(16, 3), RSP_CFG(24, 3), RSP_CFG(32, 3), RSP_CFG(40, 3),
+    RSP_CFG(48, 3),
+    RSP_CFG(8, 4),  RSP_CFG(16, 4), RSP_CFG(24, 4), RSP_CFG(32, 4), RSP_CFG(40, 4),
 };
+#undef RSP_CFG
 constexpr int kNumCfg = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
 
-int choose_cfg(int need) {
+// smallest instantiation with NP parts whose part capacity holds `need` slots
+int choose_cfg(int NP, int need) {
   for (int i = 0; i < kNumCfg; ++i)
-    if (need <= kGatherCfgs[i].P) return i;
+    if (kGatherCfgs[i].NP == NP && need <= kGatherCfgs[i].S) return i;
   return -1;
 }
 
@@ -821,16 +843,13 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   // lanes per conflict group: 32 banks -> half waves, 64 -> full waves, 0 -> no scheduling
   const int G = NB == 0 ? 64 : NB;
   const int64_t nw0 = ceil_div(N, 64);
-  // per wave, per part: slot assignments
-  std::vector<std::vector<std::vector<int>>> wave_slots(nw0);  // [w][lane] -> entries by slot
-  std::vector<std::vector<int>> wave_pend(nw0, std::vector<int>(kNPMax, 0));
-  int need = 0;
+  // per wave and part: slot -> LDS entry (within the part) per lane, -1 = free slot
+  std::vector<std::vector<std::vector<std::vector<int>>>> placed(nw0);
+  std::vector<int> need(nw0, 0);
   for (int64_t w = 0; w < nw0; ++w) {
-    std::vector<std::vector<int>> lane_slots(64);  // slot -> LDS entry (within part), -1 pad
-    int kbase = 0;
+    placed[w].assign(e->NP, std::vector<std::vector<int>>(64));
     for (int p = 0; p < e->NP; ++p) {
-      int part_slots = 0;
-      std::vector<std::vector<int>> placed(64);  // per lane: slot -> entry
+      int ns_max = 0;
       for (int g0 = 0; g0 < 64; g0 += G) {
         std::vector<std::vector<int>> ent(G);
         for (int l = 0; l < G; ++l) {
@@ -847,96 +866,89 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
         }
         std::vector<std::vector<int>> slot_of;
         const int ns = schedule_group(ent, NB, slot_of);
-        part_slots = std::max(part_slots, ns);
+        ns_max = std::max(ns_max, ns);
         for (int l = 0; l < G; ++l) {
-          auto& pl = placed[g0 + l];
+          auto& pl = placed[w][p][g0 + l];
           pl.assign(ns, -1);
           for (size_t r = 0; r < ent[l].size(); ++r) pl[slot_of[l][r]] = ent[l][r];
         }
       }
-      part_slots = (int)ceil_div(part_slots, 8) * 8;
-      const int part_len = std::min(kPartMax, e->n_scored - p * kPartMax);
-      for (int g0 = 0; g0 < 64; g0 += G) {
-        for (int k = 0; k < part_slots; ++k) {
-          uint64_t used = 0;
-          for (int l = 0; l < G && NB > 0; ++l) {
-            auto& pl = placed[g0 + l];
-            if (k < (int)pl.size() && pl[k] >= 0) used |= 1ull << (pl[k] % NB);
-          }
-          for (int l = 0; l < G; ++l) {
-            auto& pl = placed[g0 + l];
-            if ((int)pl.size() <= k) pl.resize(k + 1, -1);
-            if (pl[k] >= 0) continue;
-            if (NB == 0) {
-              pl[k] = part_len + (l % 32);  // zero pads
-              continue;
-            }
-            int bank = 0;
-            while (bank < NB && ((used >> bank) & 1ull)) ++bank;
-            if (bank == NB) bank = 0;
-            used |= 1ull << bank;
-            pl[k] = part_len + ((bank - part_len) % NB + NB) % NB;  // a zero pad entry
-          }
-        }
-      }
-      for (int l = 0; l < 64; ++l)
-        for (int k = 0; k < part_slots; ++k) lane_slots[l].push_back(placed[l][k]);
-      kbase += part_slots;
-      wave_pend[w][p] = kbase;
+      need[w] = std::max(need[w], (int)ceil_div(ns_max, 8) * 8);
     }
-    for (int p = e->NP; p < kNPMax; ++p) wave_pend[w][p] = kbase;
-    wave_slots[w] = std::move(lane_slots);
-    need = std::max(need, kbase);
   }
-  // slot classes: each wave gets the smallest instantiation that holds its slots; the
-  // choice is made non-increasing along the (descending) wave order so that classes are
-  // contiguous runs of waves
+  // waves in descending order of slot need, then classes = runs of equal instantiation
+  std::vector<int64_t> worder(nw0);
+  std::iota(worder.begin(), worder.end(), 0);
+  std::stable_sort(worder.begin(), worder.end(),
+                   [&](int64_t a, int64_t b) { return need[a] > need[b]; });
   std::vector<int> wcfg(nw0);
-  for (int64_t w = nw0 - 1; w >= 0; --w) {
-    int c = choose_cfg(std::max(wave_pend[w][kNPMax - 1], 16));
-    if (c < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs more than 160 slots");
-    if (w + 1 < nw0) c = std::max(c, wcfg[w + 1]);
-    wcfg[w] = c;
+  for (int64_t i = 0; i < nw0; ++i) {
+    const int c = choose_cfg(e->NP, std::max(need[worder[i]], 8));
+    if (c < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs too many slots");
+    wcfg[i] = c;
   }
   e->classes.clear();
   int64_t gw_total = 0, slot_total = 0;
-  for (int64_t w = 0; w < nw0;) {
-    int64_t w1 = w;
-    while (w1 < nw0 && wcfg[w1] == wcfg[w]) ++w1;
+  for (int64_t i = 0; i < nw0;) {
+    int64_t i1 = i;
+    while (i1 < nw0 && wcfg[i1] == wcfg[i]) ++i1;
     rsp_emc::Class c;
-    c.cfg = wcfg[w];
+    c.cfg = wcfg[i];
     const GatherCfg& g = kGatherCfgs[c.cfg];
-    c.w_src = w;
-    c.n_src = w1 - w;
+    c.w_src = i;
+    c.n_src = i1 - i;
     c.n_tiles = (int)ceil_div(c.n_src, g.NW);
     c.gw0 = (int)gw_total;
     c.slot_off = slot_total;
     gw_total += (int64_t)c.n_tiles * g.NW;
-    slot_total += (int64_t)c.n_tiles * g.NW * g.P * 64;
+    slot_total += (int64_t)c.n_tiles * g.NW * g.S * g.NP * 64;
     e->classes.push_back(c);
-    w = w1;
+    i = i1;
   }
   const int64_t n_waves = gw_total;
   const int64_t Npad = n_waves * 64;
   std::vector<uint32_t> slots((size_t)slot_total);
-  std::vector<int32_t> meta((size_t)(n_waves * kNPMax), 0);
   std::vector<int32_t> perm(Npad, -1);
-  const uint32_t never_read = (uint32_t)std::min(kPartMax, e->n_scored);
   for (const auto& c : e->classes) {
     const GatherCfg& g = kGatherCfgs[c.cfg];
+    const int P = g.S * g.NP;
     for (int64_t lw = 0; lw < (int64_t)c.n_tiles * g.NW; ++lw) {
-      const int64_t gw = c.gw0 + lw, w = c.w_src + lw;
+      const int64_t gw = c.gw0 + lw;
       const bool real = lw < c.n_src;
-      for (int p = 0; p < kNPMax; ++p) meta[gw * kNPMax + p] = real ? wave_pend[w][p] : 0;
-      for (int l = 0; l < 64; ++l) {
+      const int64_t w = real ? worder[c.w_src + lw] : -1;
+      for (int l = 0; l < 64; ++l)
         if (real && w * 64 + l < N) perm[gw * 64 + l] = order[w * 64 + l];
-        for (int k = 0; k < g.P; ++k) {
-          int entry;
-          if (real && k < (int)wave_slots[w][l].size())
-            entry = wave_slots[w][l][k];
-          else
-            entry = (int)never_read + (l % 32);  // a zero pad (never read if k >= end)
-          slots[(size_t)(c.slot_off + ((int64_t)lw * g.P + k) * 64 + l)] = (uint32_t)entry * 4u;
+      for (int p = 0; p < g.NP; ++p) {
+        const int part_len = std::min(kPartMax, e->n_scored - p * kPartMax);
+        for (int g0 = 0; g0 < 64; g0 += G) {
+          for (int k = 0; k < g.S; ++k) {
+            // banks used by real entries in this slot, then conflict-free zero pads
+            uint64_t used = 0;
+            for (int l = 0; l < G && NB > 0 && real; ++l) {
+              const auto& pl = placed[w][p][g0 + l];
+              if (k < (int)pl.size() && pl[k] >= 0) used |= 1ull << (pl[k] % NB);
+            }
+            for (int l = 0; l < G; ++l) {
+              int entry = -1;
+              if (real) {
+                const auto& pl = placed[w][p][g0 + l];
+                if (k < (int)pl.size()) entry = pl[k];
+              }
+              if (entry < 0) {
+                if (NB == 0) {
+                  entry = part_len + (l % 32);
+                } else {
+                  int bank = 0;
+                  while (bank < NB && ((used >> bank) & 1ull)) ++bank;
+                  if (bank == NB) bank = 0;
+                  used |= 1ull << bank;
+                  entry = part_len + ((bank - part_len) % NB + NB) % NB;
+                }
+              }
+              slots[(size_t)(c.slot_off + ((int64_t)lw * P + p * g.S + k) * 64 + g0 + l)] =
+                  (uint32_t)entry * 4u;
+            }
+          }
         }
       }
     }
@@ -944,13 +956,13 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   int rc = 0;
   std::vector<double> hb(hbg);
   if ((rc = dev_upload(&e->d_bg, hb)) || (rc = dev_upload(&e->d_slots, slots)) ||
-      (rc = dev_upload(&e->d_meta, meta)) || (rc = dev_upload(&e->d_perm, perm)) ||
+      (rc = dev_upload(&e->d_perm, perm)) ||
       (rc = dev_upload(&e->d_ex_off, ex_off)) || (rc = dev_upload(&e->d_ex_n1, ex_n1)) ||
       (rc = dev_upload(&e->d_ex_pix, ex_pix)) || (rc = dev_upload(&e->d_ex_cnt, ex_cnt)) ||
       (rc = dev_alloc(&e->d_scale_sorted, Npad)) || (rc = dev_alloc(&e->d_running, Npad)) ||
       (rc = dev_alloc(&e->d_full_list, Npad)))
     return rc;
-  e->P = kGatherCfgs[e->classes.front().cfg].P;
+  e->P = kGatherCfgs[e->classes.front().cfg].S * kGatherCfgs[e->classes.front().cfg].NP;
   e->NW = kGatherCfgs[e->classes.front().cfg].NW;
   e->n_waves = (int)n_waves;
   e->Npad = Npad;
@@ -1036,9 +1048,9 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
         sm.gw_end[ci] = c.gw0 + c.n_tiles * gc.NW;
         sm.nseg[ci] = ns;
         hipLaunchKernelGGL(gc.fn, dim3((unsigned)(ns * c.n_tiles)), dim3(64 * gc.NW), lds_bytes,
-                           st, e->d_tables, e->TS, e->NP, (int)hb, nb, ns, c.n_tiles, c.gw0,
-                           e->d_slots + c.slot_off, e->d_meta, e->d_scale_sorted, e->d_init,
-                           e->R, e->d_partials, (int)e->Npad);
+                           st, e->d_tables, e->TS, (int)hb, nb, ns, c.n_tiles, c.gw0,
+                           e->d_slots + c.slot_off, e->d_scale_sorted, e->d_init, e->R,
+                           e->d_partials, (int)e->Npad);
       }
       RSP_LAUNCH_CHECK("k_gather");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));

@@ -137,3 +137,34 @@ def test_estep_batching_and_segments_do_not_change_result(small_frames, small_mo
         emc.set_option(4, nseg)
         emc._calculate_prob(m)
         assert np.array_equal(emc._get_rmax(), r0), (batch, nseg)
+
+
+@pytest.mark.parametrize('size', [141, 205, 256])
+def test_estep_multi_part_tables(size):
+    """Scored annuli larger than one LDS part (2, 3 and 4 table parts) match the oracle."""
+    import torch
+    from oracle import emc_oracle as O
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.emc import EMC
+    cfg = synth.SynthConfig(size=size, num_data=40, mean_count=400., n_blobs=20,
+                            disk_radius=8., beamstop=4., shift_max=1., seed=size)
+    fr = synth.make_frames(cfg)
+    grid = dict(shiftx=(-1., 1., 2), shifty=(0., 0., 1), sphere_dia=(7., 9., 2))
+    rng = np.random.default_rng(size)
+    model = 0.05 * (rng.random(size * size) + 1j * rng.random(size * size))
+    emc = EMC(dataset=Dataset(frames=fr), size=size, num_rot=5, model=model, save=False,
+              **grid)
+    emc._calculate_prob(torch.as_tensor(model, device='cuda'))
+    emc._get_rmax()
+    st = emc.estep_stats()
+    assert st['table_parts'] == {141: 2, 205: 3, 256: 4}[size]
+    ofr = O.SparseFramesNP(fr.ones, fr.multi, fr.place_ones, fr.place_multi, fr.count_multi,
+                           fr.num_pix)
+    sx, sy, dia, ns = O.state_grid(grid['shiftx'], grid['shifty'], grid['sphere_dia'])
+    prob, msums, vscale = O.estep_prob(model, size, (sx, sy, dia), 5, ofr)
+    mx, rmax = O.get_rmax(prob)
+    margin = O.top_margin(prob)
+    bad = np.nonzero(emc.rmax != rmax)[0]
+    assert np.all(margin[bad] < 1e-9 * np.abs(mx[bad])), (bad, margin[bad])
+    np.testing.assert_allclose(emc.maxll, mx, rtol=1e-12, atol=1e-9)
