@@ -225,7 +225,7 @@ __device__ __forceinline__ float* part_buf() {
 // q: wait until its part is in LDS (this wave's loads via vmcnt, every wave's via the
 // barrier), start the loads of step q+2 into the buffer freed by step q-1, then gather the
 // part's S slots.  Template: S slots per part and lane, NP parts, NW waves, L loads/wave.
-template <int S, int NP, int NW, int L>
+template <int S, int NP, int NW, int L, int X>
 struct GatherStep {
   static constexpr int P = S * NP;
   const float* tables;
@@ -271,7 +271,12 @@ struct GatherStep {
     gather_range<PP * S, PP * S + S, P>(reinterpret_cast<const char*>(part_buf<QS % 3>()), a,
                                         acc);
     if constexpr (PP == NP - 1) {
-      if (h < h1) top3_insert(top, (acc[0] + acc[1]) + (acc[2] + acc[3]), h);
+      double tot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+      // a frame split over X lanes: sum the X partial sums (only its first lane carries
+      // the init term; every lane of the group then holds the same total)
+#pragma unroll
+      for (int off = 1; off < X; off <<= 1) tot += __shfl_xor(tot, off);
+      if (h < h1) top3_insert(top, tot, h);
     }
   }
 
@@ -286,7 +291,7 @@ struct GatherStep {
   }
 };
 
-template <int S, int NP, int NW, int L>
+template <int S, int NP, int NW, int L, int X>
 __global__ __launch_bounds__(64 * NW) void k_gather(
     const float* __restrict__ tables, int TS, int hb, int nb, int nseg, int n_tiles, int gw0,
     const uint32_t* __restrict__ slots, const double* __restrict__ scale_sorted,
@@ -316,7 +321,7 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   // every wave issues exactly L 1 KB direct-to-LDS loads per step (surplus loads repeat the
   // last chunk with identical bytes), so "vmcnt <= L" means: this wave's loads of the
   // older step have landed
-  const GatherStep<S, NP, NW, L> st{tables, TS, hb, h0, h1, (h1 - h0) * NP, R, wid, lane,
+  const GatherStep<S, NP, NW, L, X> st{tables, TS, hb, h0, h1, (h1 - h0) * NP, R, wid, lane,
                                     scale_sorted[j]};
   st.issue(0, g_buf0);
   st.issue(1, g_buf1);
@@ -654,29 +659,43 @@ constexpr int nw_of(int P, int NP) {
 constexpr int l_of(int P, int NP) { return (kChunks + nw_of(P, NP) - 1) / nw_of(P, NP); }
 
 struct GatherCfg {
-  int S, NP, NW;
+  int S, NP, NW, X;
   GatherFn fn;
 };
 
 #define RSP_CFG(S, NP) \
-  {S, NP, nw_of(S * NP, NP), &k_gather<S, NP, nw_of(S * NP, NP), l_of(S * NP, NP)>}
+  {S, NP, nw_of(S * NP, NP), 1, &k_gather<S, NP, nw_of(S * NP, NP), l_of(S * NP, NP), 1>}
+#define RSP_CFGX(S, NP, X) \
+  {S, NP, nw_of(S * NP, NP), X, &k_gather<S, NP, nw_of(S * NP, NP), l_of(S * NP, NP), X>}
 const GatherCfg kGatherCfgs[] = {
     RSP_CFG(16, 1), RSP_CFG(32, 1), RSP_CFG(48, 1), RSP_CFG(64, 1), RSP_CFG(80, 1),
     RSP_CFG(96, 1), RSP_CFG(112, 1), RSP_CFG(128, 1), RSP_CFG(160, 1),
     RSP_CFG(8, 2),  RSP_CFG(16, 2), RSP_CFG(24, 2), RSP_CFG(32, 2), RSP_CFG(40, 2),
-    RSP_CFG(48, 2), RSP_CFG(56, 2), RSP_CFG(64, 2), RSP_CFG(80, 2),
-    RSP_CFG(8, 3),  RSP_CFG(16, 3), RSP_CFG(24, 3), RSP_CFG(32, 3), RSP_CFG(40, 3),
-    RSP_CFG(48, 3),
-    RSP_CFG(8, 4),  RSP_CFG(16, 4), RSP_CFG(24, 4), RSP_CFG(32, 4), RSP_CFG(40, 4),
+    RSP_CFG(48, 2), RSP_CFG(56, 2), RSP_CFG(64, 2),
+    RSP_CFG(8, 3),  RSP_CFG(16, 3), RSP_CFG(24, 3), RSP_CFG(32, 3),
+    RSP_CFG(8, 4),  RSP_CFG(16, 4), RSP_CFG(24, 4),
+    // frames split over X lanes
+    RSP_CFGX(96, 1, 2), RSP_CFGX(96, 1, 4), RSP_CFGX(96, 1, 8),
+    RSP_CFGX(40, 2, 2), RSP_CFGX(40, 2, 4), RSP_CFGX(40, 2, 8),
+    RSP_CFGX(24, 3, 2), RSP_CFGX(24, 3, 4), RSP_CFGX(24, 3, 8),
+    RSP_CFGX(16, 4, 2), RSP_CFGX(16, 4, 4), RSP_CFGX(16, 4, 8),
 };
 #undef RSP_CFG
+#undef RSP_CFGX
 constexpr int kNumCfg = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
 
-// smallest instantiation with NP parts whose part capacity holds `need` slots
-int choose_cfg(int NP, int need) {
+// smallest instantiation with NP parts and lane split X whose part capacity holds `need`
+int choose_cfg(int NP, int X, int need) {
   for (int i = 0; i < kNumCfg; ++i)
-    if (kGatherCfgs[i].NP == NP && need <= kGatherCfgs[i].S) return i;
+    if (kGatherCfgs[i].NP == NP && kGatherCfgs[i].X == X && need <= kGatherCfgs[i].S) return i;
   return -1;
+}
+
+int max_slots(int NP, int X) {
+  int m = 0;
+  for (int i = 0; i < kNumCfg; ++i)
+    if (kGatherCfgs[i].NP == NP && kGatherCfgs[i].X == X) m = std::max(m, kGatherCfgs[i].S);
+  return m;
 }
 
 }  // namespace
@@ -842,48 +861,75 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   const int NB = e->NB;
   // lanes per conflict group: 32 banks -> half waves, 64 -> full waves, 0 -> no scheduling
   const int G = NB == 0 ? 64 : NB;
-  const int64_t nw0 = ceil_div(N, 64);
-  // per wave and part: slot -> LDS entry (within the part) per lane, -1 = free slot
-  std::vector<std::vector<std::vector<std::vector<int>>>> placed(nw0);
-  std::vector<int> need(nw0, 0);
-  for (int64_t w = 0; w < nw0; ++w) {
-    placed[w].assign(e->NP, std::vector<std::vector<int>>(64));
+  // Waves of 64 lanes over the sorted frames.  A frame whose entries do not fit the largest
+  // instantiation is split over X lanes (X = 2, 4, 8; entry r goes to lane r % X of its
+  // group), each wave then holding 64/X frames.  Per wave and part: slot -> LDS entry
+  // (within the part) per lane, -1 = free slot.
+  struct Wave {
+    int X;
+    int64_t f0, nf;  // frames order[f0 .. f0+nf)
+    int need;        // slots per part (max over parts, multiple of 8)
+    std::vector<std::vector<std::vector<int>>> placed;  // [part][lane][slot]
+  };
+  std::vector<Wave> waves;
+  auto build_wave = [&](int64_t f0, int X, Wave& wv) {
+    wv.X = X;
+    wv.f0 = f0;
+    wv.nf = std::min<int64_t>(64 / X, N - f0);
+    wv.need = 0;
+    wv.placed.assign(e->NP, std::vector<std::vector<int>>(64));
     for (int p = 0; p < e->NP; ++p) {
       int ns_max = 0;
       for (int g0 = 0; g0 < 64; g0 += G) {
         std::vector<std::vector<int>> ent(G);
         for (int l = 0; l < G; ++l) {
-          const int64_t jj = w * 64 + g0 + l;
-          if (jj >= N) continue;
-          const int d = order[jj];
+          const int lane = g0 + l, k = lane / X;
+          if (k >= wv.nf) continue;
+          const int d = order[f0 + k];
+          int r = 0;
           for (int64_t q = ex_off[d]; q < ex_off[d + 1]; ++q) {
             const int ci = e->compact_of[ex_pix[q]];
-            if (ci / kPartMax != p) continue;
-            const int local = ci - p * kPartMax;
             const int c = (q < ex_off[d] + ex_n1[d]) ? 1 : ex_cnt[q];
-            for (int r = 0; r < c; ++r) ent[l].push_back(local);
+            for (int m = 0; m < c; ++m, ++r)
+              if (ci / kPartMax == p && r % X == lane % X) ent[l].push_back(ci - p * kPartMax);
           }
         }
         std::vector<std::vector<int>> slot_of;
         const int ns = schedule_group(ent, NB, slot_of);
         ns_max = std::max(ns_max, ns);
         for (int l = 0; l < G; ++l) {
-          auto& pl = placed[w][p][g0 + l];
+          auto& pl = wv.placed[p][g0 + l];
           pl.assign(ns, -1);
           for (size_t r = 0; r < ent[l].size(); ++r) pl[slot_of[l][r]] = ent[l][r];
         }
       }
-      need[w] = std::max(need[w], (int)ceil_div(ns_max, 8) * 8);
+      wv.need = std::max(wv.need, (int)ceil_div(ns_max, 8) * 8);
     }
+  };
+  for (int64_t f0 = 0; f0 < N;) {
+    Wave wv;
+    int X = 1;
+    for (;; X *= 2) {
+      build_wave(f0, X, wv);
+      if (wv.need <= max_slots(e->NP, X)) break;
+      if (X == 8)
+        return fail(kErrUnsupported, "rsp_emc_set_frames: a frame has too many scored photons");
+    }
+    f0 += wv.nf;
+    waves.push_back(std::move(wv));
   }
-  // waves in descending order of slot need, then classes = runs of equal instantiation
+  const int64_t nw0 = (int64_t)waves.size();
+  // waves in descending order of (lane split, slot need); classes = runs of equal
+  // instantiation
   std::vector<int64_t> worder(nw0);
   std::iota(worder.begin(), worder.end(), 0);
-  std::stable_sort(worder.begin(), worder.end(),
-                   [&](int64_t a, int64_t b) { return need[a] > need[b]; });
+  std::stable_sort(worder.begin(), worder.end(), [&](int64_t a, int64_t b) {
+    return waves[a].X != waves[b].X ? waves[a].X > waves[b].X : waves[a].need > waves[b].need;
+  });
   std::vector<int> wcfg(nw0);
   for (int64_t i = 0; i < nw0; ++i) {
-    const int c = choose_cfg(e->NP, std::max(need[worder[i]], 8));
+    const Wave& wv = waves[worder[i]];
+    const int c = choose_cfg(e->NP, wv.X, std::max(wv.need, 8));
     if (c < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs too many slots");
     wcfg[i] = c;
   }
@@ -915,9 +961,11 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
     for (int64_t lw = 0; lw < (int64_t)c.n_tiles * g.NW; ++lw) {
       const int64_t gw = c.gw0 + lw;
       const bool real = lw < c.n_src;
-      const int64_t w = real ? worder[c.w_src + lw] : -1;
+      const Wave* wv = real ? &waves[worder[c.w_src + lw]] : nullptr;
+      // the first lane of each frame's group stands for the frame; the others are -1
       for (int l = 0; l < 64; ++l)
-        if (real && w * 64 + l < N) perm[gw * 64 + l] = order[w * 64 + l];
+        if (real && l % wv->X == 0 && l / wv->X < wv->nf)
+          perm[gw * 64 + l] = order[wv->f0 + l / wv->X];
       for (int p = 0; p < g.NP; ++p) {
         const int part_len = std::min(kPartMax, e->n_scored - p * kPartMax);
         for (int g0 = 0; g0 < 64; g0 += G) {
@@ -925,13 +973,13 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
             // banks used by real entries in this slot, then conflict-free zero pads
             uint64_t used = 0;
             for (int l = 0; l < G && NB > 0 && real; ++l) {
-              const auto& pl = placed[w][p][g0 + l];
+              const auto& pl = wv->placed[p][g0 + l];
               if (k < (int)pl.size() && pl[k] >= 0) used |= 1ull << (pl[k] % NB);
             }
             for (int l = 0; l < G; ++l) {
               int entry = -1;
               if (real) {
-                const auto& pl = placed[w][p][g0 + l];
+                const auto& pl = wv->placed[p][g0 + l];
                 if (k < (int)pl.size()) entry = pl[k];
               }
               if (entry < 0) {
