@@ -1,0 +1,9 @@
+"""Drop-in module ``emc`` of the reference (same import name): re-exports
+``refspi_amd.emc.EMC`` so ``from emc import EMC`` resolves to the MI355X path."""
+import os as _os
+import sys as _sys
+
+_sys.path.insert(0, _os.path.dirname(_os.path.dirname(_os.path.abspath(__file__))))
+
+from refspi_amd.emc import *  # noqa: F401,F403,E402
+from refspi_amd.emc import EMC  # noqa: E402,F401

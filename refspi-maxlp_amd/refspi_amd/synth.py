@@ -229,3 +229,41 @@ def make_frames_torch(cfg: SynthConfig, device='cuda', batch: int = 1024):
                 place_ones=torch.cat(po_l), place_multi=torch.cat(pm_l),
                 count_multi=torch.cat(cm_l), angles=angles, diameters=dia, shifts=shifts,
                 fluence=fl)
+
+
+PRESETS = {'cpu64': CONFIG_CPU64, '185': CONFIG_185, '256': CONFIG_256}
+
+
+def write_photons_file(path, cfg: SynthConfig):
+    """Generate ``cfg`` with the NumPy back end and write a photons file in the reference
+    layout (make_data.py:228-275: sparse photons, ``num_pix``, ``scale``, ``true_*``)."""
+    from . import h5io
+    fr = make_frames(cfg)
+    t = fr.truth
+    h5io.write_photons(path, fr, extra={'scale': t['fluence'], 'true_shifts': t['shifts'],
+                                        'true_diameters': t['diameters'],
+                                        'true_angles': t['angles']})
+    return fr
+
+
+def main(argv=None):
+    import argparse
+    import dataclasses
+    ap = argparse.ArgumentParser(description='Synthetic holographic SPI photons file')
+    ap.add_argument('out', help='output HDF5 photons file')
+    ap.add_argument('--preset', choices=sorted(PRESETS), default='cpu64')
+    ap.add_argument('--num-data', type=int, default=None)
+    ap.add_argument('--seed', type=int, default=None)
+    a = ap.parse_args(argv)
+    cfg = PRESETS[a.preset]
+    if a.num_data is not None:
+        cfg = dataclasses.replace(cfg, num_data=a.num_data)
+    if a.seed is not None:
+        cfg = dataclasses.replace(cfg, seed=a.seed)
+    fr = write_photons_file(a.out, cfg)
+    print('%d frames, %.1f photons/frame -> %s' %
+          (fr.num_data, (fr.place_ones.size + fr.count_multi.sum()) / fr.num_data, a.out))
+
+
+if __name__ == '__main__':
+    main()
