@@ -136,6 +136,57 @@ int rsp_maxlp_info(const rsp_maxlp* plan, int64_t* out);
 
 int rsp_maxlp_destroy(rsp_maxlp* plan);
 
+/* ---------------------------------------------------------------------------------
+ * Rotation form (per-frame shift/diameter, rotations as the latent variable): the
+ * dense cross term sum_p O_r(p) conj(R_d(p)) as a GEMM.  No reference counterpart: the
+ * reference's nearest code is the per-state unrotated msums of emc.py:172-181.
+ * ------------------------------------------------------------------------------- */
+
+/* C[d*M + r] = sum_k A[r*K + k] * B[d*K + k] (row-major, device pointers).
+ *   mode 0: FP32 in/out through rocBLAS sgemm (the plain FP32 form)
+ *   mode 1: FP32 in/out, split-BF16 hi/lo products with FP32 accumulation on MFMA
+ *   mode 2: FP64 in/out through rocBLAS dgemm
+ * Mode 1 allocates its padded bf16 planes on `stream` for the call. */
+int rsp_xterm_gemm(int mode, int64_t M, int64_t N, int64_t K, const void* A, const void* B,
+                   void* C, rsp_stream_t stream);
+
+/* Rotation-form plan over the scored pixels (probmask == 0 of emc.py:89-98):
+ *   O_r = bilinear rotation of the complex model by the r-th angle (kernels.cu:6-42
+ *         geometry), R_d = rel_scale * sphere(dia_d) * shift ramp(sx_d, sy_d) (the
+ *         reference term of kernels.cu:44-82), W = scale_d * |O_r + R_d|^2,
+ *   LL[d*R + r] = sum_photons K log W - sum_pixels W (log clamp of kernels.cu:37-40),
+ *   prob[d*R + r] = softmax_r LL (FP64), logz[d] = log sum_r exp LL.
+ * rot_cos/rot_sin: host, num_rot (num_rot <= 768). */
+typedef struct rsp_rot rsp_rot;
+int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_cos,
+                   const double* rot_sin, double rel_scale, rsp_rot** out);
+/* Sparse frames (host arrays, rsp_emc_set_frames layout); photons outside the scored
+ * pixels are dropped.  Synchronous. */
+int rsp_rot_set_frames(rsp_rot* plan, int64_t num_data, const int32_t* ones,
+                       const int32_t* multi, const int32_t* place_ones,
+                       const int32_t* place_multi, const int32_t* count_multi);
+/* Keys: 1 = cross-term precision (0 FP32 rocBLAS, 1 split BF16 MFMA = default, 2 FP64
+ * with an FP64 log term), 2 = frames per chunk (0 = about 8 GB of operand rows),
+ * 5 = per-kernel timing (read with rsp_rot_timing). */
+int rsp_rot_set_option(rsp_rot* plan, int key, int64_t value);
+/* model_c: device c16 npix; sx, sy, dia, scales: device f64 N; ll, prob: device f64 N*R
+ * (either may be NULL); logz: device f64 N (may be NULL).  Synchronises `stream`. */
+int rsp_rot_estep(rsp_rot* plan, const double* model_c, const double* sx, const double* sy,
+                  const double* dia, const double* scales, double* ll, double* prob,
+                  double* logz, rsp_stream_t stream);
+/* Merged intensity model (device f64 npix) from prob (device f64 N*R): for every
+ * rotation merge_all (kernels.cu:139-152) then slice_merge (kernels.cu:84-114), divided
+ * by the slice_merge weights times sum_d prob*scale; 0 where the weight is 0.  acc_out
+ * and weights_out (device f64 npix) may be NULL.  Synchronises `stream`. */
+int rsp_rot_merge(rsp_rot* plan, const double* prob, const double* scales, double* model_out,
+                  double* acc_out, double* weights_out, rsp_stream_t stream);
+/* host double[5]: ms of slices, reference rows, GEMM, log term + LSE; chunks. */
+int rsp_rot_timing(const rsp_rot* plan, double* out);
+/* host int64[7]: frames, scored pixels, padded K, padded rotations, chunk, photon
+ * entries in the scored pixels, distinct radii. */
+int rsp_rot_info(const rsp_rot* plan, int64_t* out);
+int rsp_rot_destroy(rsp_rot* plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,339 @@
+// Dense cross term of the rotation form on MFMA:
+//   X[d, r] = sum_k A[r, k] * B[d, k]
+// with A = the rotated model slices [Re O_r(p), Im O_r(p)] and B = the per-frame
+// reference terms [Re R_d(p), Im R_d(p)] over the scored pixels, so that
+// X = Re sum_p O_r(p) conj(R_d(p)).
+//
+// Split-BF16: each FP32 operand x is carried as hi = bf16(x), lo = bf16(x - hi); the
+// product uses hi*hi + hi*lo + lo*hi (the lo*lo term, below 2^-16 relative, is dropped)
+// with FP32 accumulation in v_mfma_f32_32x32x16_bf16.
+//
+// Kernel layout (k_gemm_split): 256 threads = 4 waves, macro tile 256 (r) x 128 (d),
+// each wave 128 x 64 = 4 x 2 MFMA tiles of 32 x 32.  K advances in stages of 32 bf16
+// through a ring of three static LDS buffers (48 KB per stage) filled by direct
+// global->LDS loads two stages ahead.  LDS rows are 64 B (4 slots of 16 B) with the
+// slot XOR-swizzled by f(row) = ((row >> 1) ^ (row >> 3)) & 3, which makes the
+// fragment reads (ds_read_b128, lane -> row lane%32, slot 2*kk + lane/32) hit distinct
+// banks for every 8-, 16- and 32-lane group.  Work-groups are mapped XCD-major so the
+// r-tiles of one d-tile run on one XCD and share its L2.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "refspi_amd.h"
+#include "rsp_common.hpp"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTM = 256, kTN = 128, kBK = 32;
+constexpr int kRowB = kBK * 2;                 // bytes per tile row (64)
+constexpr int kTileA = kTM * kRowB;            // 16 KB per A plane tile
+constexpr int kTileB = kTN * kRowB;            // 8 KB per B plane tile
+constexpr int kLA = kTileA / 1024 / 4;         // DMA instructions per wave per A plane (4)
+constexpr int kLB = kTileB / 1024 / 4;         // per B plane (2)
+constexpr int kL = 2 * (kLA + kLB);            // loads per wave per stage (12)
+static_assert(kL < 16, "vmcnt immediate");
+
+__shared__ __attribute__((aligned(16))) char s_ah0[kTileA], s_ah1[kTileA], s_ah2[kTileA];
+__shared__ __attribute__((aligned(16))) char s_al0[kTileA], s_al1[kTileA], s_al2[kTileA];
+__shared__ __attribute__((aligned(16))) char s_bh0[kTileB], s_bh1[kTileB], s_bh2[kTileB];
+__shared__ __attribute__((aligned(16))) char s_bl0[kTileB], s_bl1[kTileB], s_bl2[kTileB];
+
+template <int S> __device__ __forceinline__ char* t_ah() {
+  if constexpr (S == 0) return s_ah0; else if constexpr (S == 1) return s_ah1; else return s_ah2;
+}
+template <int S> __device__ __forceinline__ char* t_al() {
+  if constexpr (S == 0) return s_al0; else if constexpr (S == 1) return s_al1; else return s_al2;
+}
+template <int S> __device__ __forceinline__ char* t_bh() {
+  if constexpr (S == 0) return s_bh0; else if constexpr (S == 1) return s_bh1; else return s_bh2;
+}
+template <int S> __device__ __forceinline__ char* t_bl() {
+  if constexpr (S == 0) return s_bl0; else if constexpr (S == 1) return s_bl1; else return s_bl2;
+}
+
+__device__ __forceinline__ int swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+
+__device__ __forceinline__ void dma16(const void* g, char* l) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)g,
+                                   (__attribute__((address_space(3))) void*)l, 16, 0, 0);
+}
+
+struct SplitGemm {
+  const __bf16 *ah, *al, *bh, *bl;   // planes: A [Mpad x Kp], B [Npad x Kp]
+  int64_t aoff[kLA], boff[kLB];      // per-lane element offsets of this tile's slots
+  int nk, wave, lane;
+
+  template <int S>
+  __device__ __forceinline__ void issue(int q) const {
+    const int64_t k0 = (int64_t)min(q, nk - 1) * kBK;
+#pragma unroll
+    for (int u = 0; u < kLA; ++u) {
+      dma16(ah + aoff[u] + k0, t_ah<S>() + (wave * kLA + u) * 1024);
+      dma16(al + aoff[u] + k0, t_al<S>() + (wave * kLA + u) * 1024);
+    }
+#pragma unroll
+    for (int u = 0; u < kLB; ++u) {
+      dma16(bh + boff[u] + k0, t_bh<S>() + (wave * kLB + u) * 1024);
+      dma16(bl + boff[u] + k0, t_bl<S>() + (wave * kLB + u) * 1024);
+    }
+  }
+
+  template <int S>
+  __device__ __forceinline__ void compute(f32x16 (&acc)[4][2], int wm, int wn) const {
+    const int lr = lane & 31;
+    const int rowoff = lr * kRowB;
+    const int fs = swz(lr);
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+      const int kc = ((2 * kk + (lane >> 5)) ^ fs) * 16;
+      bf16x8 a_h[4], a_l[4], b_h[2], b_l[2];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int o = (wm * 128 + i * 32) * kRowB + rowoff + kc;
+        a_h[i] = *reinterpret_cast<const bf16x8*>(t_ah<S>() + o);
+        a_l[i] = *reinterpret_cast<const bf16x8*>(t_al<S>() + o);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int o = (wn * 64 + j * 32) * kRowB + rowoff + kc;
+        b_h[j] = *reinterpret_cast<const bf16x8*>(t_bh<S>() + o);
+        b_l[j] = *reinterpret_cast<const bf16x8*>(t_bl<S>() + o);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l[i], b_h[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_l[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_h[j], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+
+  template <int S>
+  __device__ __forceinline__ void step(int q, f32x16 (&acc)[4][2], int wm, int wn) const {
+    // the loads of stage q are the older kL of the 2*kL in flight
+    __builtin_amdgcn_s_waitcnt(0xF70 | kL);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    issue<(S + 2) % 3>(q + 2);
+    if (q < nk) compute<S>(acc, wm, wn);
+  }
+};
+
+__global__ __launch_bounds__(256, 1) void k_gemm_split(
+    const __bf16* __restrict__ ah, const __bf16* __restrict__ al,
+    const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, float* __restrict__ C,
+    int64_t N, int64_t Kp, int64_t ldc, int nM, int nN) {
+  const int T = nM * nN;
+  const int T8 = (T + 7) / 8;
+  const int task = (int)(blockIdx.x & 7) * T8 + (int)(blockIdx.x >> 3);
+  if (task >= T) return;
+  const int mt = task % nM, nt = task / nM;
+  const int64_t m0 = (int64_t)mt * kTM, n0 = (int64_t)nt * kTN;
+  SplitGemm g;
+  g.ah = ah; g.al = al; g.bh = bh; g.bl = bl;
+  g.lane = threadIdx.x & 63;
+  g.wave = threadIdx.x >> 6;
+  g.nk = (int)(Kp / kBK);
+#pragma unroll
+  for (int u = 0; u < kLA; ++u) {
+    const int s = (g.wave * kLA + u) * 64 + g.lane;
+    const int row = s >> 2, kcl = (s & 3) ^ swz(row);
+    g.aoff[u] = (m0 + row) * Kp + kcl * 8;
+  }
+#pragma unroll
+  for (int u = 0; u < kLB; ++u) {
+    const int s = (g.wave * kLB + u) * 64 + g.lane;
+    const int row = s >> 2, kcl = (s & 3) ^ swz(row);
+    g.boff[u] = (n0 + row) * Kp + kcl * 8;
+  }
+  const int wm = g.wave >> 1, wn = g.wave & 1;
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
+  g.issue<0>(0);
+  g.issue<1>(1);
+  const int ng = (g.nk + 2) / 3;
+  for (int grp = 0; grp < ng; ++grp) {
+    const int q = grp * 3;
+    g.step<0>(q, acc, wm, wn);
+    g.step<1>(q + 1, acc, wm, wn);
+    g.step<2>(q + 2, acc, wm, wn);
+  }
+  __builtin_amdgcn_s_waitcnt(0xF70);   // the trailing (clamped) loads
+  // C[d, r]: lane -> d = lane%32 of the 32-wide tile, rows 8*(e/4) + 4*(lane/32) + e%4
+  const int lr = g.lane & 31, hi = g.lane >> 5;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int64_t d = n0 + wn * 64 + j * 32 + lr;
+    if (d >= N) continue;
+    float* crow = C + d * ldc + m0 + wm * 128;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e4 = 0; e4 < 4; ++e4) {
+        f32x4 v = {acc[i][j][4 * e4], acc[i][j][4 * e4 + 1], acc[i][j][4 * e4 + 2],
+                   acc[i][j][4 * e4 + 3]};
+        *reinterpret_cast<f32x4*>(crow + i * 32 + 8 * e4 + 4 * hi) = v;
+      }
+  }
+}
+
+// FP32 rows [rows x K] (leading dimension ld) -> bf16 hi/lo planes [rows_pad x Kp],
+// zero padded.  One thread per 8 consecutive k.
+__global__ void k_split_rows(const float* __restrict__ X, int64_t rows, int64_t K, int64_t ld,
+                             __bf16* __restrict__ hi, __bf16* __restrict__ lo,
+                             int64_t rows_pad, int64_t Kp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per_row = Kp / 8;
+  if (t >= rows_pad * per_row) return;
+  const int64_t r = t / per_row, k0 = (t - r * per_row) * 8;
+  bf16x8 h, l;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float x = (r < rows && k0 + e < K) ? X[r * ld + k0 + e] : 0.f;
+    const __bf16 xh = (__bf16)x;
+    h[e] = xh;
+    l[e] = (__bf16)(x - (float)xh);
+  }
+  *reinterpret_cast<bf16x8*>(hi + r * Kp + k0) = h;
+  *reinterpret_cast<bf16x8*>(lo + r * Kp + k0) = l;
+}
+
+rocblas_handle g_blas = nullptr;
+int g_blas_dev = -1;
+
+int blas_handle(hipStream_t stream, rocblas_handle* out) {
+  int dev = 0;
+  RSP_HIP(hipGetDevice(&dev));
+  if (g_blas == nullptr || g_blas_dev != dev) {
+    if (g_blas) rocblas_destroy_handle(g_blas);
+    g_blas = nullptr;
+    if (rocblas_create_handle(&g_blas) != rocblas_status_success)
+      return rsp::fail(rsp::kErrState, "rocblas_create_handle failed");
+    g_blas_dev = dev;
+  }
+  if (rocblas_set_stream(g_blas, stream) != rocblas_status_success)
+    return rsp::fail(rsp::kErrState, "rocblas_set_stream failed");
+  *out = g_blas;
+  return rsp::kOk;
+}
+
+}  // namespace
+
+namespace rsp {
+
+int64_t xterm_pad_m(int64_t M) { return ceil_div(M, kTM) * kTM; }
+int64_t xterm_pad_n(int64_t N) { return ceil_div(N, kTN) * kTN; }
+int64_t xterm_pad_k(int64_t K) { return ceil_div(K, kBK) * kBK; }
+
+int xterm_split(const float* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi, __bf16* lo,
+                int64_t rows_pad, int64_t Kp, hipStream_t stream) {
+  const int64_t n = rows_pad * (Kp / 8);
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, X,
+                     rows, K, ld, hi, lo, rows_pad, Kp);
+  RSP_LAUNCH_CHECK("k_split_rows");
+  return kOk;
+}
+
+// Split-BF16 GEMM on padded planes: C[d * ldc + r] for d < N, r < Mpad.
+int xterm_gemm_split(const __bf16* ah, const __bf16* al, const __bf16* bh, const __bf16* bl,
+                     float* C, int64_t Mpad, int64_t N, int64_t Kp, int64_t ldc,
+                     hipStream_t stream) {
+  if (Mpad % kTM || Kp % kBK || ldc < Mpad || ldc % 4) return fail(kErrSize, "gemm padding");
+  if (N == 0 || Mpad == 0) return kOk;
+  const int nM = (int)(Mpad / kTM), nN = (int)ceil_div(N, kTN);
+  const int T8 = (nM * nN + 7) / 8;
+  hipLaunchKernelGGL(k_gemm_split, dim3(8 * T8), dim3(256), 0, stream, ah, al, bh, bl, C, N,
+                     Kp, ldc, nM, nN);
+  RSP_LAUNCH_CHECK("k_gemm_split");
+  return kOk;
+}
+
+// FP32 / FP64 through rocBLAS: C[d * ldc + r] = sum_k A[r * lda + k] B[d * ldb + k].
+int xterm_gemm_blas32(const float* A, const float* B, float* C, int64_t M, int64_t N,
+                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream) {
+  rocblas_handle h;
+  int rc = blas_handle(stream, &h);
+  if (rc) return rc;
+  const float one = 1.f, zero = 0.f;
+  rocblas_status s = rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none,
+                                   (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &one, A,
+                                   (rocblas_int)lda, B, (rocblas_int)ldb, &zero, C,
+                                   (rocblas_int)ldc);
+  if (s != rocblas_status_success) return fail(kErrState, "rocblas_sgemm failed");
+  return kOk;
+}
+
+int xterm_gemm_blas64(const double* A, const double* B, double* C, int64_t M, int64_t N,
+                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream) {
+  rocblas_handle h;
+  int rc = blas_handle(stream, &h);
+  if (rc) return rc;
+  const double one = 1., zero = 0.;
+  rocblas_status s = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none,
+                                   (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &one, A,
+                                   (rocblas_int)lda, B, (rocblas_int)ldb, &zero, C,
+                                   (rocblas_int)ldc);
+  if (s != rocblas_status_success) return fail(kErrState, "rocblas_dgemm failed");
+  return kOk;
+}
+
+}  // namespace rsp
+
+extern "C" {
+
+int rsp_xterm_gemm(int mode, int64_t M, int64_t N, int64_t K, const void* A, const void* B,
+                   void* C, rsp_stream_t stream_) {
+  using namespace rsp;
+  hipStream_t stream = (hipStream_t)stream_;
+  if (M < 0 || N < 0 || K < 0) return fail(kErrSize, "negative size");
+  if ((M && N && K) && (!A || !B || !C)) return fail(kErrNull, "null buffer");
+  if (M > 0x7fffffff || N > 0x7fffffff || K > 0x7fffffff) return fail(kErrSize, "size too large");
+  if (M == 0 || N == 0) return kOk;
+  if (mode == 0)
+    return xterm_gemm_blas32((const float*)A, (const float*)B, (float*)C, M, N, K, K, K, M,
+                             stream);
+  if (mode == 2)
+    return xterm_gemm_blas64((const double*)A, (const double*)B, (double*)C, M, N, K, K, K, M,
+                             stream);
+  if (mode != 1) return fail(kErrUnsupported, "mode must be 0 (FP32), 1 (split BF16), 2 (FP64)");
+  const int64_t Mp = xterm_pad_m(M), Np = xterm_pad_n(N), Kp = xterm_pad_k(K);
+  __bf16* planes = nullptr;
+  float* Cp = nullptr;
+  const size_t nplane = (size_t)(2 * Mp + 2 * Np) * Kp;
+  RSP_HIP(hipMallocAsync((void**)&planes, nplane * sizeof(__bf16), stream));
+  hipError_t e = hipMallocAsync((void**)&Cp, (size_t)N * Mp * sizeof(float), stream);
+  if (e != hipSuccess) {
+    (void)hipFreeAsync(planes, stream);
+    return hip_fail(e, "hipMallocAsync");
+  }
+  __bf16 *ah = planes, *al = ah + Mp * Kp, *bh = al + Mp * Kp, *bl = bh + Np * Kp;
+  int rc = xterm_split((const float*)A, M, K, K, ah, al, Mp, Kp, stream);
+  if (!rc) rc = xterm_split((const float*)B, N, K, K, bh, bl, Np, Kp, stream);
+  if (!rc) rc = xterm_gemm_split(ah, al, bh, bl, Cp, Mp, N, Kp, Mp, stream);
+  if (!rc) {
+    hipError_t ce = hipMemcpy2DAsync(C, M * sizeof(float), Cp, Mp * sizeof(float),
+                                     M * sizeof(float), N, hipMemcpyDeviceToDevice, stream);
+    if (ce != hipSuccess) rc = hip_fail(ce, "hipMemcpy2DAsync");
+  }
+  (void)hipFreeAsync(Cp, stream);
+  (void)hipFreeAsync(planes, stream);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Cross-term GEMM entry points shared by the rotation-form plan (xterm.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rsp {
+
+// Padding the split kernel needs: rows of A to 256, rows of B to 128, K to 32.
+int64_t xterm_pad_m(int64_t M);
+int64_t xterm_pad_n(int64_t N);
+int64_t xterm_pad_k(int64_t K);
+
+// FP32 rows -> bf16 hi/lo planes (zero padded to rows_pad x Kp).
+int xterm_split(const float* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi, __bf16* lo,
+                int64_t rows_pad, int64_t Kp, hipStream_t stream);
+
+// C[d * ldc + r] = sum_k A[r, k] B[d, k] on padded bf16 hi/lo planes (split products).
+int xterm_gemm_split(const __bf16* ah, const __bf16* al, const __bf16* bh, const __bf16* bl,
+                     float* C, int64_t Mpad, int64_t N, int64_t Kp, int64_t ldc,
+                     hipStream_t stream);
+
+// The same through rocBLAS in FP32 / FP64 (row-major A [M x lda], B [N x ldb]).
+int xterm_gemm_blas32(const float* A, const float* B, float* C, int64_t M, int64_t N,
+                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream);
+int xterm_gemm_blas64(const double* A, const double* B, double* C, int64_t M, int64_t N,
+                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream);
+
+}  // namespace rsp

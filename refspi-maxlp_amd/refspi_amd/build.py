@@ -19,7 +19,7 @@ INCLUDE = os.path.join(os.path.dirname(ROOT), 'include')
 BUILD_DIR = os.path.join(ROOT, '_build')
 LIB_PATH = os.path.join(PKG_DIR, 'librefspi_amd.so')
 
-SOURCES = ['rsp_status.cpp', 'mirror.hip', 'estep.hip', 'maxlp.hip']
+SOURCES = ['rsp_status.cpp', 'mirror.hip', 'estep.hip', 'maxlp.hip', 'xterm.hip', 'rotem.hip']
 
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-std=c++17', '-ffp-contract=off',
@@ -45,7 +45,8 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB_PATH + '.tmp'
-    cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC'] + objs + ['-o', tmp]
+    cmd = [HIPCC, '--offload-arch=gfx950', '-shared', '-fPIC'] + objs + [
+        '-L/opt/rocm/lib', '-Wl,-rpath,/opt/rocm/lib', '-lrocblas', '-o', tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError('link failed:\n' + r.stderr)
