@@ -73,6 +73,133 @@ def cpu_baseline(fr, size, num_rot, grid, n_frames, n_states):
                        (n_frames, n_states, num_rot, el))
 
 
+def _frames_from_torch(d):
+    return types.SimpleNamespace(
+        num_pix=d['num_pix'], bg=None,
+        **{k: d[k].cpu().numpy().astype('i4') for k in
+           ('ones', 'multi', 'place_ones', 'place_multi', 'count_multi')})
+
+
+def _scored(size):
+    from refspi_amd.emc import probmask
+    return np.nonzero(probmask(size) == 0)[0]
+
+
+def split_leg(args):
+    """Rotation-form E-step (cross-term GEMM + sparse log term + FP64 log-sum-exp) at
+    256x256, 200,000 frames (log-normal fluence, sigma 0.5, mean 400 photons), 720
+    rotations: split-BF16 (MFMA) vs plain FP32 (rocBLAS) vs FP64, the LL error of each
+    against the FP64 mode, and the time of each."""
+    import torch
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.rotem import RotEM
+    cfg = synth.SynthConfig(**{**synth.CONFIG_256.__dict__, 'num_data': args.split_frames,
+                               'seed': args.seed})
+    R = args.split_rot
+    t = time.time()
+    d = synth.make_frames_torch(cfg, device='cuda')
+    fr = _frames_from_torch(d)
+    pix = _scored(cfg.size)
+    alpha, rel = synth.model_scales(cfg, pix)
+    model = torch.as_tensor(alpha * synth.object_ft(cfg), device='cuda')
+    sx, sy = d['shifts'][:, 0].contiguous(), d['shifts'][:, 1].contiguous()
+    dia, sc = d['diameters'], d['fluence']
+    plan = RotEM(Dataset(frames=fr), cfg.size, R, rel_scale=rel, mode='split')
+    plan.set_timing(True)
+    info = plan.info()
+    log('split leg: %d frames generated, plan built in %.1f s %s' %
+        (cfg.num_data, time.time() - t, info))
+    N, K = cfg.num_data, info['scored_pixels']
+    out = dict(workload='rotation-form E-step, %dx%d q-grid, %d frames, log-normal fluence '
+                        '(sigma %.1f, mean %d photons), %d rotations, %d scored pixels' %
+                        (cfg.size, cfg.size, N, cfg.fluence_sigma, cfg.mean_count, R, K))
+    # Re(O conj(R)) needs 4 real flops per (r, d, pixel): a real GEMM of M = R, N frames,
+    # K = 2 x scored pixels (re/im interleaved), 2 M N K flops
+    useful = 4. * R * N * K
+    res = {}
+    for mode in ('fp64', 'fp32', 'split'):
+        plan.set_mode(mode)
+        plan.estep_device(model, sx, sy, dia, sc, want_ll=False, want_prob=False)  # warm-up
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(args.split_reps if mode != 'fp64' else 1):
+            t = time.time()
+            ll, _, logz = plan.estep_device(model, sx, sy, dia, sc, want_ll=True,
+                                            want_prob=False)
+            torch.cuda.synchronize()
+            el = time.time() - t
+            tm = plan.timing()
+            if best is None or el < best[0]:
+                best = (el, tm)
+        el, tm = best
+        res[mode] = ll
+        gemm_s = tm['gemm_ms'] / 1e3
+        out[mode] = dict(estep_s=el, gemm_ms=tm['gemm_ms'], slices_ms=tm['slices_ms'],
+                         ref_rows_ms=tm['ref_rows_ms'], logterm_lse_ms=tm['logterm_lse_ms'],
+                         gemm_tflops=useful / gemm_s / 1e12,
+                         frame_rotation_evals_per_s=N * R / el)
+        log('split leg %s: %.3f s %s' % (mode, el, tm))
+    ll64 = res['fp64']
+    for mode in ('fp32', 'split'):
+        err = ((res[mode] - ll64).abs() / ll64.abs()).max().item()
+        out[mode]['ll_max_rel_err_vs_fp64'] = err
+    out['split_speedup_vs_fp32'] = dict(estep=out['fp32']['estep_s'] / out['split']['estep_s'],
+                                        gemm=out['fp32']['gemm_ms'] / out['split']['gemm_ms'])
+    out['gemm'] = dict(M=R, N=N, K=2 * K, flops=useful,
+                       note='cross term Re sum_p O_r(p) conj(R_d(p)) as a real GEMM over '
+                            're/im-interleaved rows (K = 2 x scored pixels): 2 M N K flops; '
+                            'split-BF16 executes 3 BF16 products per term (3x these flops on '
+                            'MFMA)')
+    out['split']['mfma_bf16_tflops_executed'] = 3. * useful / (out['split']['gemm_ms'] / 1e3) / 1e12
+    out['peaks_tflops'] = dict(bf16_dense=2500., fp32_vector=157.3, fp64=78.6)
+    del plan, res
+    torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_leg(args):
+    """Loss + gradient of the rotation form (rsp_rot_estep then rsp_rot_adjoint) at
+    185x185, 20,000 frames, 360 rotations, gradients with respect to the complex model and
+    every frame's shift and diameter."""
+    import torch
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.rotem import RotEM
+    cfg = synth.SynthConfig(**{**synth.CONFIG_185.__dict__, 'num_data': args.adj_frames,
+                               'seed': args.seed})
+    R = 360
+    d = synth.make_frames_torch(cfg, device='cuda')
+    fr = _frames_from_torch(d)
+    pix = _scored(cfg.size)
+    alpha, rel = synth.model_scales(cfg, pix)
+    model = torch.as_tensor(alpha * synth.object_ft(cfg), device='cuda')
+    sx, sy = d['shifts'][:, 0].contiguous(), d['shifts'][:, 1].contiguous()
+    dia = d['diameters']
+    plan = RotEM(Dataset(frames=fr), cfg.size, R, rel_scale=rel, mode='fp64')
+    out = dict(workload='loss + gradient (model, shifts, diameters), %dx%d q-grid, %d frames, '
+                        '%d rotations' % (cfg.size, cfg.size, cfg.num_data, R))
+    for mode in ('fp64', 'split'):
+        plan.set_mode(mode)
+        plan.loss_grad(model, sx, sy, dia)           # warm-up (allocations)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t = time.time()
+            _, prob, logz = plan.estep_device(model, sx, sy, dia, want_ll=False)
+            torch.cuda.synchronize()
+            t1 = time.time()
+            plan.adjoint_device(model, sx, sy, dia, None, prob)
+            torch.cuda.synchronize()
+            ts.append((time.time() - t, t1 - t, time.time() - t1))
+        best = min(ts)
+        out['forward_' + mode] = dict(total_s=best[0], forward_s=best[1], adjoint_s=best[2])
+        log('adjoint leg (%s forward): %s' % (mode, best))
+    del plan
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -86,6 +213,13 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-frames', type=int, default=1000)
     ap.add_argument('--cpu-states', type=int, default=10)
+    ap.add_argument('--workload', choices=('all', 'joint', 'split', 'adjoint'), default='all',
+                    help='joint: the EMC+MaxLP iteration (the headline value); split: the '
+                         'rotation-form split-BF16 vs FP32 E-step; adjoint: loss + gradient')
+    ap.add_argument('--split-frames', type=int, default=200_000)
+    ap.add_argument('--split-rot', type=int, default=720)
+    ap.add_argument('--split-reps', type=int, default=3)
+    ap.add_argument('--adj-frames', type=int, default=20_000)
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy 1 GiB device-to-device (HBM counter calibration) and exit')
     args = ap.parse_args()
@@ -105,6 +239,10 @@ def main():
     import torch
     from refspi_amd.dset import Dataset
     from refspi_amd.emc import EMC
+    if args.workload in ('split', 'adjoint'):
+        leg = split_leg(args) if args.workload == 'split' else adjoint_leg(args)
+        print(json.dumps({args.workload: leg}), flush=True)
+        return
     grid = ((-4., 4., 17), (-4., 4., 17), (7., 9., 21))
     cfg, fr = make_dataset(args)
     size = cfg.size
@@ -176,6 +314,11 @@ def main():
         lds=dict(achieved=lds_gbs, peak=LDS_PEAK_GBS, unit='GB/s', frac=lds_gbs / LDS_PEAK_GBS,
                  what='scored-photon gathers (4 B each) per step / gather time'),
     )
+    if args.workload == 'all':
+        del emc
+        torch.cuda.empty_cache()
+        out['split_precision'] = split_leg(args)
+        out['adjoint'] = adjoint_leg(args)
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(fr, size, args.num_rot, grid, args.cpu_frames,
                                            args.cpu_states)

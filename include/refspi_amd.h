@@ -180,6 +180,19 @@ int rsp_rot_estep(rsp_rot* plan, const double* model_c, const double* sx, const 
  * and weights_out (device f64 npix) may be NULL.  Synchronises `stream`. */
 int rsp_rot_merge(rsp_rot* plan, const double* prob, const double* scales, double* model_out,
                   double* acc_out, double* weights_out, rsp_stream_t stream);
+/* Gradient of L = -sum_d log((1/R) sum_r exp LL[d, r]) (uniform rotation prior) at the
+ * given inputs, in FP64, recomputing O_r and R_d (nothing of the forward pass is kept but
+ * prob).  prob: device f64 N*R, the softmax of LL of rsp_rot_estep at the same inputs.
+ * Outputs (device): g_model c16 npix as dL/dRe + i dL/dIm; g_sx, g_sy, g_dia f64 N.
+ * dL/dW = -P (K/W - 1) (only -P where W < 1e-20, the log clamp of kernels.cu:37-40),
+ * scattered through |O_r + R_d|^2, the sphere/ramp term of kernels.cu:44-82 and the
+ * transpose of the bilinear rotation of kernels.cu:6-42 (slice_merge's scatter,
+ * kernels.cu:84-114, in gather form).  No reference counterpart: maxLP.py:199-214
+ * (get_grad_pixel, never called) is the nearest code.  Synchronises `stream`. */
+int rsp_rot_adjoint(rsp_rot* plan, const double* model_c, const double* sx, const double* sy,
+                    const double* dia, const double* scales, const double* prob,
+                    double* g_model, double* g_sx, double* g_sy, double* g_dia,
+                    rsp_stream_t stream);
 /* host double[5]: ms of slices, reference rows, GEMM, log term + LSE; chunks. */
 int rsp_rot_timing(const rsp_rot* plan, double* out);
 /* host int64[7]: frames, scored pixels, padded K, padded rotations, chunk, photon

@@ -394,6 +394,284 @@ __global__ void k_merge_model(int size, int R, int Rs, const double* __restrict_
   model[m] = wts > 0. ? acc / wts : 0.;
 }
 
+// ---------------------------------------------------------------------------------
+// Adjoint of L = -sum_d log((1/R) sum_r exp LL_dr) (recomputes O_r and R_d; FP64).
+//   dL/dW_drj = -P_dr (K_dj / W_drj - 1)        (-P_dr * -1 where W < 1e-20: log clamp)
+//   dW/dtheta = 2 scale_d Re(conj(Z) dZ/dtheta),   Z = O_r(j) + R_d(j)
+// Model:  g_O_r(j) = -2 sum_{d in photons(j)} P_dr K Z / |Z|^2 + 2 ws_r O_r(j)
+//                    + 2 sum_d P_dr scale_d R_d(j)          (ws_r = sum_d P_dr scale_d)
+//         g_model  = sum_r Interp_r^T g_O_r               (slice_merge's transpose form)
+// Frame:  g_theta_d = -2 sum_r P_dr sum_photons K Re(conj(Z) dR) / |Z|^2
+//                    + 2 scale_d sum_j Re(conj(Obar_d(j) + R_d(j)) dR_d(j)),
+//         Obar_d = sum_r P_dr O_r                          (sum_r P_dr = 1)
+// ---------------------------------------------------------------------------------
+
+// Q[dl * R + r] = P[(d0 + dl) * R + r] * scale[d0 + dl]
+__global__ void k_scaled_prob(const double* __restrict__ P, const double* __restrict__ scales,
+                              int64_t d0, int64_t n, int R, double* __restrict__ Q) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * R) return;
+  const int64_t dl = i / R;
+  Q[i] = P[d0 * R + i] * scales[d0 + dl];
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Per-frame tables of the reference term: S(s_i), dS/ddia at every distinct radius, and the
+// separable phase factors exp(2 pi i (x-c) sx / size), exp(2 pi i (y-c) sy / size).
+struct RefTables {
+  double* S;
+  double* Sd;
+  double2* ex;
+  double2* ey;
+};
+
+__device__ __forceinline__ RefTables ref_tables(double* sh, int nrad, int size) {
+  RefTables t;
+  t.S = sh;
+  t.Sd = sh + nrad;
+  t.ex = reinterpret_cast<double2*>(sh + ((2 * nrad + 1) & ~1));
+  t.ey = t.ex + size;
+  return t;
+}
+
+__device__ void fill_ref_tables(RefTables tb, const double* __restrict__ rad, int nrad, int size,
+                                double rel, double fsx, double fsy, double fd) {
+  const int cen = size / 2;
+  for (int i = threadIdx.x; i < nrad; i += blockDim.x) {
+    double s = rad[i] * kPi * fd / size;
+    if (s == 0.) s = 1.e-5;
+    double sn, cs;
+    sincos(s, &sn, &cs);
+    const double sv = (sn - s * cs) / (s * s * s);
+    tb.S[i] = rel * sv;
+    // dS/ds = sin(s)/s^2 - 3 S(s)/s ; ds/ddia = pi rad / size
+    tb.Sd[i] = rel * (sn / (s * s) - 3. * sv / s) * (kPi * rad[i] / size);
+  }
+  for (int i = threadIdx.x; i < size; i += blockDim.x) {
+    double sn, cs;
+    sincos(2. * kPi * ((double)(i - cen) * fsx) / size, &sn, &cs);
+    tb.ex[i] = make_double2(cs, sn);
+    sincos(2. * kPi * ((double)(i - cen) * fsy) / size, &sn, &cs);
+    tb.ey[i] = make_double2(cs, sn);
+  }
+}
+
+// R_d(j) and its derivatives by sx, sy, dia at scored pixel t (radius index jr).
+__device__ __forceinline__ void ref_and_derivs(const RefTables& tb, int t, int jr, int size,
+                                               double2& Rv, double2& Dx, double2& Dy,
+                                               double2& Dd) {
+  const int cen = size / 2;
+  const int x = t / size, y = t - (t / size) * size;
+  const double2 e = cmul(tb.ex[x], tb.ey[y]);
+  const double sv = tb.S[jr], sd = tb.Sd[jr];
+  Rv = make_double2(sv * e.x, sv * e.y);
+  Dd = make_double2(sd * e.x, sd * e.y);
+  const double kx = 2. * kPi * (double)(x - cen) / size;
+  const double ky = 2. * kPi * (double)(y - cen) / size;
+  Dx = make_double2(-kx * Rv.y, kx * Rv.x);    // i kx R
+  Dy = make_double2(-ky * Rv.y, ky * Rv.x);    // i ky R
+}
+
+__device__ __forceinline__ double block_sum256(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+// One work-group per frame of [d0, d0 + gridDim.x): frame gradients (sx, sy, dia).
+// Dense part: threads over the scored pixels (Obar row of the chunk).  Sparse part: wave w
+// takes photons w, w+4, ...; lanes over rotations r = lane + 64 t.
+template <int NT>
+__global__ __launch_bounds__(256) void k_adj_frame(
+    int size, int K, int R, int Rs, double rel, const int32_t* __restrict__ scored,
+    const int32_t* __restrict__ jrad, const double* __restrict__ rad, int nrad,
+    const int64_t* __restrict__ fo, const int32_t* __restrict__ fj,
+    const float* __restrict__ fk, const double2* __restrict__ OT, const double* __restrict__ P,
+    const double* __restrict__ Obar, int64_t Kp, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ dia,
+    const double* __restrict__ scales, int64_t d0, double* __restrict__ gsx,
+    double* __restrict__ gsy, double* __restrict__ gdia) {
+  extern __shared__ double sh[];
+  __shared__ double red[4];
+  const int64_t dl = blockIdx.x, d = d0 + dl;
+  const double sc = scales[d];
+  const RefTables tb = ref_tables(sh, nrad, size);
+  fill_ref_tables(tb, rad, nrad, size, rel, sx[d], sy[d], dia[d]);
+  __syncthreads();
+  // dense: sum_j Re(conj(Obar + R) dR)
+  double ex_ = 0., ey_ = 0., ed_ = 0.;
+  const double2* ob = reinterpret_cast<const double2*>(Obar + dl * Kp);
+  for (int j = threadIdx.x; j < K; j += 256) {
+    double2 Rv, Dx, Dy, Dd;
+    ref_and_derivs(tb, scored[j], jrad[j], size, Rv, Dx, Dy, Dd);
+    const double2 o = ob[j];
+    const double zr = o.x + Rv.x, zi = o.y + Rv.y;
+    ex_ += zr * Dx.x + zi * Dx.y;
+    ey_ += zr * Dy.x + zi * Dy.y;
+    ed_ += zr * Dd.x + zi * Dd.y;
+  }
+  // sparse: per rotation sum_photons K Re(conj(Z) dR) / |Z|^2
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double ax[NT], ay[NT], ad[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) ax[t] = ay[t] = ad[t] = 0.;
+  const int64_t o1 = fo[d + 1];
+  for (int64_t q = fo[d] + w; q < o1; q += 4) {
+    const int j = fj[q];
+    const double k = (double)fk[q];
+    double2 Rv, Dx, Dy, Dd;
+    ref_and_derivs(tb, scored[j], jrad[j], size, Rv, Dx, Dy, Dd);
+    const double2* row = OT + (int64_t)j * Rs + lane;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double2 o = row[64 * t];
+      const double zr = o.x + Rv.x, zi = o.y + Rv.y;
+      const double n2 = zr * zr + zi * zi;
+      if (sc * n2 >= 1.e-20) {
+        const double f = k / n2;
+        ax[t] += f * (zr * Dx.x + zi * Dx.y);
+        ay[t] += f * (zr * Dy.x + zi * Dy.y);
+        ad[t] += f * (zr * Dd.x + zi * Dd.y);
+      }
+    }
+  }
+  double px = 0., py = 0., pd = 0.;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int r = lane + 64 * t;
+    if (r < R) {
+      const double pr = P[d * R + r];
+      px += pr * ax[t];
+      py += pr * ay[t];
+      pd += pr * ad[t];
+    }
+  }
+  const double gx = -2. * block_sum256(px, red) + 2. * sc * block_sum256(ex_, red);
+  const double gy = -2. * block_sum256(py, red) + 2. * sc * block_sum256(ey_, red);
+  const double gd = -2. * block_sum256(pd, red) + 2. * sc * block_sum256(ed_, red);
+  if (threadIdx.x == 0) {
+    gsx[d] = gx;
+    gsy[d] = gy;
+    gdia[d] = gd;
+  }
+}
+
+// One work-group per scored pixel j: g_O_r(j) for every rotation (threads over r), from
+// the pixel-major photon lists, O_r(j) (OT), the dense GEMM term GT (column-major R x Kp)
+// and ws_r.  GO[j * Rs + r] (complex); rotations r >= R get 0.
+template <int NU>
+__global__ __launch_bounds__(256) void k_adj_pixel(
+    int size, int R, int Rs, double rel, const int32_t* __restrict__ scored,
+    const int64_t* __restrict__ po, const int32_t* __restrict__ pd,
+    const float* __restrict__ pk, const double2* __restrict__ OT, const double* __restrict__ P,
+    const double* __restrict__ GT, const double* __restrict__ wsum,
+    const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ dia, const double* __restrict__ scales,
+    double2* __restrict__ GO) {
+  __shared__ double s_rr[256], s_ri[256], s_k[256], s_sc[256];
+  __shared__ int32_t s_d[256];
+  const int j = blockIdx.x, tid = threadIdx.x;
+  const int t = scored[j];
+  double2 O[NU], acc[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int r = tid + 256 * u;
+    O[u] = r < Rs ? OT[(int64_t)j * Rs + r] : make_double2(0., 0.);
+    acc[u] = make_double2(0., 0.);
+  }
+  const int64_t e0 = po[j], e1 = po[j + 1];
+  for (int64_t base = e0; base < e1; base += 256) {
+    const int64_t e = base + tid;
+    if (e < e1) {
+      const int d = pd[e];
+      double rr, ri;
+      ref_photon(t, size, rel, sx[d], sy[d], dia[d], rr, ri);
+      s_rr[tid] = rr;
+      s_ri[tid] = ri;
+      s_k[tid] = (double)pk[e];
+      s_sc[tid] = scales[d];
+      s_d[tid] = d;
+    }
+    __syncthreads();
+    const int n = (int)min((int64_t)256, e1 - base);
+    for (int q = 0; q < n; ++q) {
+      const double rr = s_rr[q], ri = s_ri[q], k = s_k[q], sc = s_sc[q];
+      const double* prow = P + (int64_t)s_d[q] * R;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int r = tid + 256 * u;
+        if (r < R) {
+          const double zr = O[u].x + rr, zi = O[u].y + ri;
+          const double n2 = zr * zr + zi * zi;
+          if (sc * n2 >= 1.e-20) {
+            const double f = k * prow[r] / n2;
+            acc[u].x += f * zr;
+            acc[u].y += f * zi;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int r = tid + 256 * u;
+    if (r >= Rs) continue;
+    double2 g = make_double2(0., 0.);
+    if (r < R) {
+      const double ws = wsum[r];
+      g.x = -2. * acc[u].x + 2. * ws * O[u].x + 2. * GT[(int64_t)(2 * j) * R + r];
+      g.y = -2. * acc[u].y + 2. * ws * O[u].y + 2. * GT[(int64_t)(2 * j + 1) * R + r];
+    }
+    GO[(int64_t)j * Rs + r] = g;
+  }
+}
+
+// g_model(m) = sum_r sum_{scored view pixels j whose bilinear sample touches m} w * GO[j][r]
+// (the transpose of k_rot_slices' interpolation; gather form, one thread per model pixel).
+__global__ void k_adj_model(int size, int R, int Rs, const double* __restrict__ rcos,
+                            const double* __restrict__ rsin,
+                            const int32_t* __restrict__ compact_of,
+                            const double2* __restrict__ GO, double* __restrict__ g_model) {
+  const int npix = size * size;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= npix) return;
+  const int mx = m / size, my = m - (m / size) * size, cen = size / 2;
+  double gr = 0., gi = 0.;
+  for (int r = 0; r < R; ++r) {
+    const double ac = rcos[r], as = rsin[r];
+    const double mxc = (double)(mx - cen), myc = (double)(my - cen);
+    const int x0 = (int)rint(mxc * ac + myc * as + cen);
+    const int y0 = (int)rint(-mxc * as + myc * ac + cen);
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int x = x0 + dx, y = y0 + dy;
+        if (x < 0 || x >= size || y < 0 || y >= size) continue;
+        const int j = compact_of[x * size + y];
+        if (j < 0) continue;
+        double tx, ty;
+        rot_source(x, y, cen, ac, as, tx, ty);
+        const int ix = (int)floor(tx), iy = (int)floor(ty);
+        if (ix < 0 || ix > size - 2 || iy < 0 || iy > size - 2) continue;
+        const int ox = mx - ix, oy = my - iy;
+        if (ox < 0 || ox > 1 || oy < 0 || oy > 1) continue;
+        const double fx = tx - ix, fy = ty - iy;
+        const double cx = 1. - fx, cy = 1. - fy;
+        const double w = (ox ? fx : cx) * (oy ? fy : cy);
+        const double2 g = GO[(int64_t)j * Rs + r];
+        gr += g.x * w;
+        gi += g.y * w;
+      }
+  }
+  g_model[2 * m] = gr;
+  g_model[2 * m + 1] = gi;
+}
+
 template <typename T>
 int dalloc(T** p, int64_t n) {
   if (*p) {
@@ -434,6 +712,11 @@ struct rsp_rot {
   void *d_A = nullptr, *d_OT = nullptr, *d_B = nullptr, *d_X = nullptr;
   double *d_normO = nullptr, *d_part = nullptr, *d_normR = nullptr;
   double *d_VT = nullptr, *d_wsum = nullptr, *d_wpart = nullptr;
+  // adjoint (FP64 operands of its own, so that the forward mode's buffers stay)
+  int64_t adj_chunk = 0;
+  double *d_aA = nullptr, *d_aB = nullptr, *d_aObar = nullptr, *d_aQ = nullptr;
+  double *d_aGT = nullptr, *d_apart = nullptr, *d_anormO = nullptr, *d_anormR = nullptr;
+  double2 *d_aOT = nullptr, *d_aGO = nullptr;
   // timing
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -450,6 +733,108 @@ void free_work(rsp_rot* p) {
   p->d_normO = p->d_part = p->d_normR = nullptr;
   p->mode_alloc = -1;
   p->chunk_alloc = 0;
+}
+
+void free_adj(rsp_rot* p) {
+  void* ptrs[] = {p->d_aA, p->d_aB, p->d_aObar, p->d_aQ, p->d_aGT, p->d_apart,
+                  p->d_anormO, p->d_anormR, p->d_aOT, p->d_aGO};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  p->d_aA = p->d_aB = p->d_aObar = p->d_aQ = p->d_aGT = p->d_apart = nullptr;
+  p->d_anormO = p->d_anormR = nullptr;
+  p->d_aOT = p->d_aGO = nullptr;
+  p->adj_chunk = 0;
+}
+
+int ensure_adj(rsp_rot* p, int64_t chunk) {
+  if (p->adj_chunk == chunk) return kOk;
+  free_adj(p);
+  const int64_t ntiles = ceil_div(p->K, 64);
+  int rc;
+  if ((rc = dalloc(&p->d_aA, p->Mpad * p->Kp)) || (rc = dalloc(&p->d_aB, chunk * p->Kp)) ||
+      (rc = dalloc(&p->d_aObar, chunk * p->Kp)) || (rc = dalloc(&p->d_aQ, chunk * p->R)) ||
+      (rc = dalloc(&p->d_aGT, (int64_t)p->R * p->Kp)) ||
+      (rc = dalloc(&p->d_apart, ntiles * p->R)) || (rc = dalloc(&p->d_anormO, p->R)) ||
+      (rc = dalloc(&p->d_anormR, chunk)) ||
+      (rc = dalloc(&p->d_aOT, (int64_t)p->K * p->Rs)) ||
+      (rc = dalloc(&p->d_aGO, (int64_t)p->K * p->Rs))) {
+    free_adj(p);
+    return rc;
+  }
+  // operand padding (rows >= R of A, columns >= 2K of A and B) stays zero
+  RSP_HIP(hipMemset(p->d_aA, 0, sizeof(double) * (size_t)(p->Mpad * p->Kp)));
+  RSP_HIP(hipMemset(p->d_aB, 0, sizeof(double) * (size_t)(chunk * p->Kp)));
+  RSP_HIP(hipMemset(p->d_aOT, 0, sizeof(double2) * (size_t)((int64_t)p->K * p->Rs)));
+  p->adj_chunk = chunk;
+  return kOk;
+}
+
+template <int NT>
+void launch_adj_frame_nt(rsp_rot* p, size_t shmem, const double* prob, const double* sx,
+                         const double* sy, const double* dia, const double* scales, int64_t d0,
+                         int64_t n, double* gsx, double* gsy, double* gdia, hipStream_t st) {
+  hipLaunchKernelGGL((k_adj_frame<NT>), dim3((unsigned)n), dim3(256), shmem, st, p->size, p->K,
+                     p->R, p->Rs, p->rel, p->d_scored, p->d_jrad, p->d_rad, p->nrad, p->d_fo,
+                     p->d_fj, p->d_fk, p->d_aOT, prob, p->d_aObar, p->Kp, sx, sy, dia, scales,
+                     d0, gsx, gsy, gdia);
+}
+
+int launch_adj_frame(rsp_rot* p, size_t shmem, const double* prob, const double* sx,
+                     const double* sy, const double* dia, const double* scales, int64_t d0,
+                     int64_t n, double* gsx, double* gsy, double* gdia, hipStream_t st) {
+  switch (p->Rs / 64) {
+#define RSP_NT(k)                                                                        \
+  case k:                                                                                \
+    launch_adj_frame_nt<k>(p, shmem, prob, sx, sy, dia, scales, d0, n, gsx, gsy, gdia, st); \
+    break;
+    RSP_NT(1) RSP_NT(2) RSP_NT(3) RSP_NT(4) RSP_NT(5) RSP_NT(6) RSP_NT(7) RSP_NT(8)
+    RSP_NT(9) RSP_NT(10) RSP_NT(11) RSP_NT(12)
+#undef RSP_NT
+    default: return fail(kErrUnsupported, "rotation form supports at most 768 rotations");
+  }
+  RSP_LAUNCH_CHECK("k_adj_frame");
+  return kOk;
+}
+
+int launch_adj_pixel(rsp_rot* p, const double* prob, const double* sx, const double* sy,
+                     const double* dia, const double* scales, hipStream_t st) {
+  const dim3 g((unsigned)p->K), b(256);
+  switch (ceil_div(p->Rs, 256)) {
+    case 1:
+      hipLaunchKernelGGL(k_adj_pixel<1>, g, b, 0, st, p->size, p->R, p->Rs, p->rel, p->d_scored,
+                         p->d_po, p->d_pd, p->d_pk, p->d_aOT, prob, p->d_aGT, p->d_wsum, sx, sy,
+                         dia, scales, p->d_aGO);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_adj_pixel<2>, g, b, 0, st, p->size, p->R, p->Rs, p->rel, p->d_scored,
+                         p->d_po, p->d_pd, p->d_pk, p->d_aOT, prob, p->d_aGT, p->d_wsum, sx, sy,
+                         dia, scales, p->d_aGO);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_adj_pixel<3>, g, b, 0, st, p->size, p->R, p->Rs, p->rel, p->d_scored,
+                         p->d_po, p->d_pd, p->d_pk, p->d_aOT, prob, p->d_aGT, p->d_wsum, sx, sy,
+                         dia, scales, p->d_aGO);
+      break;
+    default: return fail(kErrUnsupported, "rotation form supports at most 768 rotations");
+  }
+  RSP_LAUNCH_CHECK("k_adj_pixel");
+  return kOk;
+}
+
+// ws_r = sum_d P[d, r] scale_d into p->d_wsum (fixed summation order).
+int launch_wsum(rsp_rot* p, const double* prob, const double* scales, hipStream_t st) {
+  const int64_t per = 1024;
+  const int64_t nb = ceil_div(p->N, per);
+  int rc;
+  if (!p->d_wsum && (rc = dalloc(&p->d_wsum, p->R))) return rc;
+  if ((rc = dalloc(&p->d_wpart, nb * p->R))) return rc;
+  hipLaunchKernelGGL(k_wsum_part, dim3((unsigned)nb, (unsigned)ceil_div(p->R, 256)), dim3(256),
+                     0, st, prob, scales, p->N, p->R, per, p->d_wpart);
+  RSP_LAUNCH_CHECK("k_wsum_part");
+  hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)ceil_div(p->R, 256)), dim3(256), 0, st,
+                     p->d_wpart, (int)nb, p->R, p->d_wsum);
+  RSP_LAUNCH_CHECK("k_norm_reduce");
+  return kOk;
 }
 
 int64_t elem_bytes(int mode) { return mode == kFP64 ? 8 : (mode == kSplit ? 4 : 4); }
@@ -482,29 +867,43 @@ int ensure_work(rsp_rot* p, int64_t chunk) {
   return kOk;
 }
 
+// O_r rows (A), the pixel-major copy (OT) and |O_r|^2 (normO) into the given buffers.
 template <int MODE>
-int launch_slices(rsp_rot* p, const double* model_c, hipStream_t st) {
+int launch_slices_to(rsp_rot* p, const double* model_c, void* A, void* OT, double* part,
+                     double* normO, hipStream_t st) {
   dim3 g((unsigned)ceil_div(p->K, 64), (unsigned)ceil_div(p->R, 32));
   hipLaunchKernelGGL(k_rot_slices<MODE>, g, dim3(256), 0, st, model_c, p->size, p->K, p->R,
-                     p->d_scored, p->d_rcos, p->d_rsin, p->d_A, p->Mpad, p->Kp, p->d_OT, p->Rs,
-                     p->d_part);
+                     p->d_scored, p->d_rcos, p->d_rsin, A, p->Mpad, p->Kp, OT, p->Rs, part);
   RSP_LAUNCH_CHECK("k_rot_slices");
   hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)ceil_div(p->R, 256)), dim3(256), 0, st,
-                     p->d_part, (int)ceil_div(p->K, 64), p->R, p->d_normO);
+                     part, (int)ceil_div(p->K, 64), p->R, normO);
   RSP_LAUNCH_CHECK("k_norm_reduce");
+  return kOk;
+}
+
+template <int MODE>
+int launch_slices(rsp_rot* p, const double* model_c, hipStream_t st) {
+  return launch_slices_to<MODE>(p, model_c, p->d_A, p->d_OT, p->d_part, p->d_normO, st);
+}
+
+// R_d rows of frames [d0, d0 + n) into B (Bpad rows per bf16 plane) and normR.
+template <int MODE>
+int launch_ref_to(rsp_rot* p, const double* sx, const double* sy, const double* dia, int64_t d0,
+                  int64_t n, void* B, int64_t Bpad, double* normR, hipStream_t st) {
+  const size_t shmem =
+      sizeof(double) * (size_t)((p->nrad + 1) & ~1) + 2 * sizeof(double2) * (size_t)p->size;
+  hipLaunchKernelGGL(k_ref_rows<MODE>, dim3((unsigned)n), dim3(256), shmem, st, p->size, p->K,
+                     p->d_scored, p->d_jrad, p->d_rad, p->nrad, p->rel, sx, sy, dia, d0, p->N, B,
+                     Bpad, p->Kp, normR);
+  RSP_LAUNCH_CHECK("k_ref_rows");
   return kOk;
 }
 
 template <int MODE>
 int launch_ref(rsp_rot* p, const double* sx, const double* sy, const double* dia, int64_t d0,
                int64_t n, hipStream_t st) {
-  const size_t shmem =
-      sizeof(double) * (size_t)((p->nrad + 1) & ~1) + 2 * sizeof(double2) * (size_t)p->size;
-  hipLaunchKernelGGL(k_ref_rows<MODE>, dim3((unsigned)n), dim3(256), shmem, st, p->size, p->K,
-                     p->d_scored, p->d_jrad, p->d_rad, p->nrad, p->rel, sx, sy, dia, d0, p->N,
-                     p->d_B, xterm_pad_n(p->chunk_alloc), p->Kp, p->d_normR);
-  RSP_LAUNCH_CHECK("k_ref_rows");
-  return kOk;
+  return launch_ref_to<MODE>(p, sx, sy, dia, d0, n, p->d_B, xterm_pad_n(p->chunk_alloc),
+                             p->d_normR, st);
 }
 
 template <typename T, int NT>
@@ -764,20 +1163,9 @@ int rsp_rot_merge(rsp_rot* p, const double* prob, const double* scales, double* 
   if (p->N == 0) return fail(kErrState, "rsp_rot_merge: no frames");
   hipStream_t st = (hipStream_t)stream_;
   RSP_HIP(hipSetDevice(p->device));
-  const int64_t per = 1024;
-  const int64_t nb = ceil_div(p->N, per);
   int rc;
-  if (!p->d_VT) {
-    if ((rc = dalloc(&p->d_VT, (int64_t)p->K * p->Rs)) || (rc = dalloc(&p->d_wsum, p->R)))
-      return rc;
-  }
-  if ((rc = dalloc(&p->d_wpart, nb * p->R))) return rc;
-  hipLaunchKernelGGL(k_wsum_part, dim3((unsigned)nb, (unsigned)ceil_div(p->R, 256)), dim3(256),
-                     0, st, prob, scales, p->N, p->R, per, p->d_wpart);
-  RSP_LAUNCH_CHECK("k_wsum_part");
-  hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)ceil_div(p->R, 256)), dim3(256), 0, st,
-                     p->d_wpart, (int)nb, p->R, p->d_wsum);
-  RSP_LAUNCH_CHECK("k_norm_reduce");
+  if (!p->d_VT && (rc = dalloc(&p->d_VT, (int64_t)p->K * p->Rs))) return rc;
+  if ((rc = launch_wsum(p, prob, scales, st))) return rc;
   hipLaunchKernelGGL(k_merge_view, dim3((unsigned)p->K), dim3(256), 0, st, p->d_po, p->d_pd,
                      p->d_pk, prob, p->R, p->Rs, p->d_VT);
   RSP_LAUNCH_CHECK("k_merge_view");
@@ -785,6 +1173,57 @@ int rsp_rot_merge(rsp_rot* p, const double* prob, const double* scales, double* 
                      p->size, p->R, p->Rs, p->d_rcos, p->d_rsin, p->d_compact_of, p->d_VT,
                      p->d_wsum, model_out, acc_out, weights_out);
   RSP_LAUNCH_CHECK("k_merge_model");
+  RSP_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int rsp_rot_adjoint(rsp_rot* p, const double* model_c, const double* sx, const double* sy,
+                    const double* dia, const double* scales, const double* prob,
+                    double* g_model, double* g_sx, double* g_sy, double* g_dia,
+                    rsp_stream_t stream_) {
+  if (!p || !model_c || !sx || !sy || !dia || !scales || !prob || !g_model || !g_sx || !g_sy ||
+      !g_dia)
+    return fail(kErrNull, "rsp_rot_adjoint: null argument");
+  if (p->N == 0) return fail(kErrState, "rsp_rot_adjoint: no frames");
+  hipStream_t st = (hipStream_t)stream_;
+  RSP_HIP(hipSetDevice(p->device));
+  const size_t shmem =
+      sizeof(double) * (size_t)((2 * p->nrad + 1) & ~1) + 2 * sizeof(double2) * (size_t)p->size;
+  int lds_max = 0;
+  RSP_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
+  if (shmem + 64 > (size_t)lds_max)
+    return fail(kErrSize, "rsp_rot_adjoint: reference tables exceed the LDS of a work-group");
+  int64_t chunk = p->chunk > 0 ? p->chunk : 4096;
+  chunk = std::min(chunk, p->N);
+  int rc = ensure_adj(p, chunk);
+  if (rc) return rc;
+  if ((rc = launch_slices_to<kFP64>(p, model_c, p->d_aA, p->d_aOT, p->d_apart, p->d_anormO, st)))
+    return rc;
+  if ((rc = launch_wsum(p, prob, scales, st))) return rc;
+  for (int64_t d0 = 0; d0 < p->N; d0 += chunk) {
+    const int64_t n = std::min(chunk, p->N - d0);
+    if ((rc = launch_ref_to<kFP64>(p, sx, sy, dia, d0, n, p->d_aB, chunk, p->d_anormR, st)))
+      return rc;
+    hipLaunchKernelGGL(k_scaled_prob, dim3((unsigned)ceil_div(n * p->R, 256)), dim3(256), 0, st,
+                       prob, scales, d0, n, p->R, p->d_aQ);
+    RSP_LAUNCH_CHECK("k_scaled_prob");
+    // GT (R x Kp, column-major) += Q^T B : GT[k * R + r] = sum_d Q[d, r] B[d, k]
+    if ((rc = xterm_dgemm(false, true, p->R, p->Kp, n, 1., p->d_aQ, p->R, p->d_aB, p->Kp,
+                          d0 == 0 ? 0. : 1., p->d_aGT, p->R, st)))
+      return rc;
+    // Obar (n x Kp, row-major) = P A : Obar[d * Kp + k] = sum_r P[d, r] A[r, k]
+    if ((rc = xterm_dgemm(false, false, p->Kp, n, p->R, 1., p->d_aA, p->Kp, prob + d0 * p->R,
+                          p->R, 0., p->d_aObar, p->Kp, st)))
+      return rc;
+    if ((rc = launch_adj_frame(p, shmem, prob, sx, sy, dia, scales, d0, n, g_sx, g_sy, g_dia,
+                               st)))
+      return rc;
+  }
+  if ((rc = launch_adj_pixel(p, prob, sx, sy, dia, scales, st))) return rc;
+  hipLaunchKernelGGL(k_adj_model, dim3((unsigned)ceil_div(p->npix, 256)), dim3(256), 0, st,
+                     p->size, p->R, p->Rs, p->d_rcos, p->d_rsin, p->d_compact_of, p->d_aGO,
+                     g_model);
+  RSP_LAUNCH_CHECK("k_adj_model");
   RSP_HIP(hipStreamSynchronize(st));
   return kOk;
 }
@@ -811,6 +1250,7 @@ int rsp_rot_destroy(rsp_rot* p) {
   if (!p) return kOk;
   (void)hipSetDevice(p->device);
   free_work(p);
+  free_adj(p);
   void* ptrs[] = {p->d_rcos, p->d_rsin, p->d_rad, p->d_scored, p->d_jrad, p->d_compact_of,
                   p->d_fo, p->d_po, p->d_fj, p->d_pd, p->d_fk, p->d_pk, p->d_VT, p->d_wsum,
                   p->d_wpart};

@@ -293,6 +293,25 @@ int xterm_gemm_blas64(const double* A, const double* B, double* C, int64_t M, in
   return kOk;
 }
 
+int xterm_dgemm(bool transA, bool transB, int64_t m, int64_t n, int64_t k, double alpha,
+                const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                double* C, int64_t ldc, hipStream_t stream) {
+  if (m > 0x7fffffff || n > 0x7fffffff || k > 0x7fffffff || lda > 0x7fffffff ||
+      ldb > 0x7fffffff || ldc > 0x7fffffff)
+    return fail(kErrSize, "dgemm size too large");
+  if (m == 0 || n == 0) return kOk;
+  rocblas_handle h;
+  int rc = blas_handle(stream, &h);
+  if (rc) return rc;
+  rocblas_status s = rocblas_dgemm(
+      h, transA ? rocblas_operation_transpose : rocblas_operation_none,
+      transB ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)m,
+      (rocblas_int)n, (rocblas_int)k, &alpha, A, (rocblas_int)lda, B, (rocblas_int)ldb, &beta,
+      C, (rocblas_int)ldc);
+  if (s != rocblas_status_success) return fail(kErrState, "rocblas_dgemm failed");
+  return kOk;
+}
+
 }  // namespace rsp
 
 extern "C" {

@@ -27,4 +27,10 @@ int xterm_gemm_blas32(const float* A, const float* B, float* C, int64_t M, int64
 int xterm_gemm_blas64(const double* A, const double* B, double* C, int64_t M, int64_t N,
                       int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream);
 
+// General FP64 GEMM in rocBLAS's column-major convention:
+//   C (m x n, ldc) = alpha * op(A) op(B) + beta * C, op = transpose when trans* is true.
+int xterm_dgemm(bool transA, bool transB, int64_t m, int64_t n, int64_t k, double alpha,
+                const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                double* C, int64_t ldc, hipStream_t stream);
+
 }  // namespace rsp

@@ -130,6 +130,41 @@ class RotEM:
     def merge(self, prob, scales=None):
         return tuple(o.cpu().numpy() for o in self.merge_device(prob, scales))
 
+    def adjoint_device(self, model, sx, sy, dia, scales, prob):
+        """Gradient of L = -sum_d log((1/R) sum_r exp LL[d, r]) (FP64, device tensors).
+
+        ``prob`` is P(r|d) of :meth:`estep_device` at the same inputs.  Returns
+        (g_model complex128 (npix,) = dL/dRe + i dL/dIm, g_sx, g_sy, g_dia (N,)).
+        """
+        import torch
+        f64 = torch.float64
+        N = self.num_data
+        m = self._dev(model, torch.complex128).reshape(-1)
+        if m.numel() != self.size ** 2:
+            raise ValueError('model must have size*size entries')
+        args = [self._dev(v, f64) for v in (sx, sy, dia)]
+        sc = self._dev(np.ones(N) if scales is None else scales, f64)
+        pr = self._dev(prob, f64)
+        if pr.numel() != N * self.num_rot:
+            raise ValueError('prob must have num_data * num_rot entries')
+        for v in args + [sc]:
+            if v.numel() != N:
+                raise ValueError('per-frame arrays must have num_data entries')
+        gm = torch.empty(self.size ** 2, dtype=torch.complex128, device=m.device)
+        g = [torch.empty(N, dtype=f64, device=m.device) for _ in range(3)]
+        _lib.call('rsp_rot_adjoint', self._h, _lib.ptr(torch.view_as_real(m)),
+                  *[_lib.ptr(v) for v in args], _lib.ptr(sc), _lib.ptr(pr),
+                  _lib.ptr(torch.view_as_real(gm)), *[_lib.ptr(v) for v in g],
+                  _lib.stream_handle())
+        return (gm, *g)
+
+    def loss_grad(self, model, sx, sy, dia, scales=None):
+        """NumPy (loss, g_model, g_sx, g_sy, g_dia) at the plan's forward precision."""
+        _, prob, logz = self.estep_device(model, sx, sy, dia, scales, want_ll=False)
+        loss = -float((logz - math.log(self.num_rot)).sum())
+        out = self.adjoint_device(model, sx, sy, dia, scales, prob)
+        return (loss,) + tuple(o.cpu().numpy() for o in out)
+
     def timing(self):
         out = np.zeros(5)
         _lib.call('rsp_rot_timing', self._h, _lib.host_ptr(out, C.c_double))

@@ -234,6 +234,38 @@ def make_frames_torch(cfg: SynthConfig, device='cuda', batch: int = 1024):
 PRESETS = {'cpu64': CONFIG_CPU64, '185': CONFIG_185, '256': CONFIG_256}
 
 
+def object_ft(cfg: SynthConfig):
+    """The generator's unrotated object transform on the q-grid (complex, size*size,
+    flat index x*size + y), from the same blob draw as :func:`make_frames` /
+    :func:`make_frames_torch` for ``cfg.seed``."""
+    rng = np.random.default_rng(cfg.seed)
+    cen, amp = _blobs(cfg, rng)
+    q = _grid(cfg.size) / cfg.box
+    qq = q[:, None] ** 2 + q[None, :] ** 2
+    env = 2 * np.pi * cfg.sigma ** 2 * np.exp(-2 * np.pi ** 2 * cfg.sigma ** 2 * qq)
+    A = np.exp(-2j * np.pi * q[:, None] * cen[None, :, 0]) * amp[None, :]
+    B = np.exp(-2j * np.pi * q[:, None] * cen[None, :, 1])
+    return (env * (A @ B.T)).ravel()
+
+
+def model_scales(cfg: SynthConfig, pix, dia=None):
+    """(object amplitude factor, reference rel_scale) that give a model |alpha F|^2 with
+    mean_count / (1 + ref_factor) photons over the pixels ``pix`` and a reference sphere of
+    diameter ``dia`` (default dia_mean) with ref_factor times that: the photon budget of the
+    generator, for rotation-form workloads whose model is the object transform."""
+    f = object_ft(cfg)
+    dia = cfg.dia_mean if dia is None else dia
+    x, y = pix // cfg.size, pix % cfg.size
+    c = cfg.size // 2
+    s = np.hypot(x - c, y - c) * np.pi * dia / cfg.size
+    s = np.where(s == 0, 1.e-5, s)
+    sph = (np.sin(s) - s * np.cos(s)) / s ** 3
+    pobj = cfg.mean_count / (1. + cfg.ref_factor)
+    alpha = np.sqrt(pobj / (np.abs(f[pix]) ** 2).sum())
+    rel = np.sqrt(cfg.ref_factor * pobj / (sph ** 2).sum())
+    return alpha, rel
+
+
 def write_photons_file(path, cfg: SynthConfig):
     """Generate ``cfg`` with the NumPy back end and write a photons file in the reference
     layout (make_data.py:228-275: sparse photons, ``num_pix``, ``scale``, ``true_*``)."""
