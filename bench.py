@@ -80,11 +80,6 @@ def _frames_from_torch(d):
            ('ones', 'multi', 'place_ones', 'place_multi', 'count_multi')})
 
 
-def _scored(size):
-    from refspi_amd.emc import probmask
-    return np.nonzero(probmask(size) == 0)[0]
-
-
 def split_leg(args):
     """Rotation-form E-step (cross-term GEMM + sparse log term + FP64 log-sum-exp) at
     256x256, 200,000 frames (log-normal fluence, sigma 0.5, mean 400 photons), 720
@@ -100,19 +95,19 @@ def split_leg(args):
     t = time.time()
     d = synth.make_frames_torch(cfg, device='cuda')
     fr = _frames_from_torch(d)
-    pix = _scored(cfg.size)
-    alpha, rel = synth.model_scales(cfg, pix)
+    alpha, rel = synth.model_scales(cfg)
     model = torch.as_tensor(alpha * synth.object_ft(cfg), device='cuda')
     sx, sy = d['shifts'][:, 0].contiguous(), d['shifts'][:, 1].contiguous()
-    dia, sc = d['diameters'], d['fluence']
-    plan = RotEM(Dataset(frames=fr), cfg.size, R, rel_scale=rel, mode='split')
+    dia, sc = d['diameters'], d['scale']
+    plan = RotEM(Dataset(frames=fr), cfg.size, R, rel_scale=rel, mode='split', full_turn=True)
     plan.set_timing(True)
     info = plan.info()
     log('split leg: %d frames generated, plan built in %.1f s %s' %
         (cfg.num_data, time.time() - t, info))
     N, K = cfg.num_data, info['scored_pixels']
     out = dict(workload='rotation-form E-step, %dx%d q-grid, %d frames, log-normal fluence '
-                        '(sigma %.1f, mean %d photons), %d rotations, %d scored pixels' %
+                        '(sigma %.1f, mean %d photons), %d rotations (full turn), %d scored '
+                        'pixels' %
                         (cfg.size, cfg.size, N, cfg.fluence_sigma, cfg.mean_count, R, K))
     # Re(O conj(R)) needs 4 real flops per (r, d, pixel): a real GEMM of M = R, N frames,
     # K = 2 x scored pixels (re/im interleaved), 2 M N K flops
@@ -171,25 +166,24 @@ def adjoint_leg(args):
     R = 360
     d = synth.make_frames_torch(cfg, device='cuda')
     fr = _frames_from_torch(d)
-    pix = _scored(cfg.size)
-    alpha, rel = synth.model_scales(cfg, pix)
+    alpha, rel = synth.model_scales(cfg)
     model = torch.as_tensor(alpha * synth.object_ft(cfg), device='cuda')
     sx, sy = d['shifts'][:, 0].contiguous(), d['shifts'][:, 1].contiguous()
-    dia = d['diameters']
-    plan = RotEM(Dataset(frames=fr), cfg.size, R, rel_scale=rel, mode='fp64')
+    dia, sc = d['diameters'], d['scale']
+    plan = RotEM(Dataset(frames=fr), cfg.size, R, rel_scale=rel, mode='fp64', full_turn=True)
     out = dict(workload='loss + gradient (model, shifts, diameters), %dx%d q-grid, %d frames, '
                         '%d rotations' % (cfg.size, cfg.size, cfg.num_data, R))
     for mode in ('fp64', 'split'):
         plan.set_mode(mode)
-        plan.loss_grad(model, sx, sy, dia)           # warm-up (allocations)
+        plan.loss_grad(model, sx, sy, dia, sc)       # warm-up (allocations)
         torch.cuda.synchronize()
         ts = []
         for _ in range(3):
             t = time.time()
-            _, prob, logz = plan.estep_device(model, sx, sy, dia, want_ll=False)
+            _, prob, logz = plan.estep_device(model, sx, sy, dia, sc, want_ll=False)
             torch.cuda.synchronize()
             t1 = time.time()
-            plan.adjoint_device(model, sx, sy, dia, None, prob)
+            plan.adjoint_device(model, sx, sy, dia, sc, prob)
             torch.cuda.synchronize()
             ts.append((time.time() - t, t1 - t, time.time() - t1))
         best = min(ts)
@@ -198,6 +192,119 @@ def adjoint_leg(args):
     del plan
     torch.cuda.empty_cache()
     return out
+
+
+def stream_leg(args):
+    """Two object classes (60/40, independent blob seeds) at 256x256, 500,000 frames
+    (log-normal fluence, sigma 0.7, mean 400 photons) packed in pinned host memory and
+    streamed in 50,000-frame chunks; 720 rotations (full turn) x 2 classes; split-BF16
+    cross term.  One E-step + merge pass, then adjoint-driven (shift, diameter) refinement
+    passes; the refinement is judged on a subset by the reference's grid search; peak HBM
+    is sampled after every chunk."""
+    import torch
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.refine import grid_search, refine
+    from refspi_amd.rotem import PASS_MERGE, StreamedRotEM
+    free0, total = torch.cuda.mem_get_info()
+    peak = [total - free0]
+
+    def sample(*_):
+        f, _t = torch.cuda.mem_get_info()
+        peak[0] = max(peak[0], total - f)
+
+    cfg = synth.SynthConfig(**{**synth.CONFIG_256.__dict__, 'num_data': args.stream_frames,
+                               'fluence_sigma': 0.7, 'class_weights': (0.6, 0.4),
+                               'seed': args.seed})
+    R, C = 720, 2
+    t = time.time()
+    d = synth.make_frames_torch(cfg, device='cuda')
+    fr = _frames_from_torch(d)
+    truth = {k: d[k].cpu().numpy() for k in ('shifts', 'diameters', 'scale', 'classes')}
+    del d
+    torch.cuda.empty_cache()
+    rel = synth.model_scales(cfg)[1]
+    models = torch.as_tensor(np.stack([synth.model_scales(cfg, c)[0] * synth.object_ft(cfg, c)
+                                       for c in range(C)]), device='cuda')
+    t_gen = time.time() - t
+    t = time.time()
+    plan = StreamedRotEM(Dataset(frames=fr), cfg.size, R, rel_scale=rel, mode='split',
+                         classes=C, full_turn=True, chunk_frames=args.stream_chunk)
+    t_pack = time.time() - t
+    N = cfg.num_data
+    log('stream leg: %d frames generated in %.1f s, packed into %d chunks (%.2f GB pinned) in '
+        '%.1f s' % (N, t_gen, len(plan.chunks), plan.host_bytes / 1e9, t_pack))
+    rng = np.random.default_rng(1)
+    sx0 = truth['shifts'][:, 0] + rng.normal(0, 0.25, N)
+    sy0 = truth['shifts'][:, 1] + rng.normal(0, 0.25, N)
+    d0 = truth['diameters'] + rng.normal(0, 0.1, N)
+    sc = truth['scale']
+    plan.run_pass(models, sx0, sy0, d0, sc, flags=PASS_MERGE)     # warm-up (allocations)
+    torch.cuda.synchronize()
+    sample()
+    t = time.time()
+    out = plan.run_pass(models, sx0, sy0, d0, sc, flags=PASS_MERGE, on_chunk=sample)
+    torch.cuda.synchronize()
+    t_pass = time.time() - t
+    sample()
+    cls = (out['best'].cpu().numpy() // R)
+    t = time.time()
+    sx, sy, dia, lz, hist = refine(plan, models, sx0, sy0, d0, sc, steps=args.refine_steps,
+                                   on_step=lambda i, h: (sample(), log('refine step', i, h)))
+    torch.cuda.synchronize()
+    t_ref = time.time() - t
+    sample()
+    res = dict(workload='two classes (60/40) at 256x256, %d frames (log-normal fluence sigma '
+                        '0.7, mean 400) streamed from pinned host memory in %d-frame chunks, '
+                        '%d rotations (full turn) x %d classes, split-BF16 cross term' %
+                        (N, args.stream_chunk, R, C),
+               estep_merge_pass_s=t_pass,
+               frame_hypothesis_evals_per_s=N * R * C / t_pass,
+               refine_steps=args.refine_steps, refine_s=t_ref,
+               refine_s_per_step=t_ref / max(1, args.refine_steps),
+               refine_history=hist,
+               class_of_best_hypothesis_correct=float((cls == truth['classes']).mean()),
+               peak_hbm_gb=peak[0] / 1e9, hbm_total_gb=total / 1e9,
+               pinned_host_gb=plan.host_bytes / 1e9, chunks=len(plan.chunks))
+    del plan
+    torch.cuda.empty_cache()
+    # the judge: the reference's grid over (shift, diameter) on a subset of frames
+    n = args.judge_frames
+    sub = types.SimpleNamespace(num_pix=fr.num_pix, bg=None)
+    oa = np.concatenate([[0], np.cumsum(fr.ones, dtype='i8')])
+    ma = np.concatenate([[0], np.cumsum(fr.multi, dtype='i8')])
+    sub.ones, sub.multi = fr.ones[:n], fr.multi[:n]
+    sub.place_ones = fr.place_ones[:oa[n]]
+    sub.place_multi, sub.count_multi = fr.place_multi[:ma[n]], fr.count_multi[:ma[n]]
+    t = time.time()
+    gsx, gsy, gd, glz, _ = grid_search(sub, cfg.size, R, rel, models, scales=sc[:n],
+                                       classes=C, full_turn=True, mode='split')
+    t_judge = time.time() - t
+    lzs = lz[:n].cpu().numpy()
+    err = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2)))
+    ts = truth['shifts'][:n].T
+
+    def serr(a, b):
+        # (rotation + pi, -shift) gives the same hologram: the error is taken to the nearer
+        # of the two twins
+        e = np.minimum(((a - b) ** 2).sum(0), ((a + b) ** 2).sum(0))
+        return float(np.sqrt(np.mean(e) / 2.))
+
+    res['judge'] = dict(
+        frames=n, grid='shift -4..4 (17) x -4..4 (17), diameter 7..9 (21): 6069 states',
+        seconds=t_judge,
+        refined_logz_ge_grid_best=float((lzs >= glz - 1e-9 * np.abs(glz)).mean()),
+        median_logz_gain_over_grid=float(np.median(lzs - glz)),
+        shift_rms_err_px=dict(start=serr(np.stack([sx0[:n], sy0[:n]]), ts),
+                              grid=serr(np.stack([gsx, gsy]), ts),
+                              refined=serr(np.stack([sx[:n].cpu().numpy(),
+                                                     sy[:n].cpu().numpy()]), ts),
+                              note='per axis, to the nearer of the twins s and -s'),
+        dia_rms_err_px=dict(start=err(d0[:n], truth['diameters'][:n]),
+                            grid=err(gd, truth['diameters'][:n]),
+                            refined=err(dia[:n].cpu().numpy(), truth['diameters'][:n])))
+    log('stream leg:', res)
+    return res
 
 
 def main():
@@ -213,7 +320,12 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-frames', type=int, default=1000)
     ap.add_argument('--cpu-states', type=int, default=10)
-    ap.add_argument('--workload', choices=('all', 'joint', 'split', 'adjoint'), default='all',
+    ap.add_argument('--stream-frames', type=int, default=500_000)
+    ap.add_argument('--stream-chunk', type=int, default=50_000)
+    ap.add_argument('--refine-steps', type=int, default=3)
+    ap.add_argument('--judge-frames', type=int, default=512)
+    ap.add_argument('--workload', choices=('all', 'joint', 'split', 'adjoint', 'stream'),
+                    default='all',
                     help='joint: the EMC+MaxLP iteration (the headline value); split: the '
                          'rotation-form split-BF16 vs FP32 E-step; adjoint: loss + gradient')
     ap.add_argument('--split-frames', type=int, default=200_000)
@@ -239,8 +351,8 @@ def main():
     import torch
     from refspi_amd.dset import Dataset
     from refspi_amd.emc import EMC
-    if args.workload in ('split', 'adjoint'):
-        leg = split_leg(args) if args.workload == 'split' else adjoint_leg(args)
+    if args.workload in ('split', 'adjoint', 'stream'):
+        leg = dict(split=split_leg, adjoint=adjoint_leg, stream=stream_leg)[args.workload](args)
         print(json.dumps({args.workload: leg}), flush=True)
         return
     grid = ((-4., 4., 17), (-4., 4., 17), (7., 9., 21))
@@ -319,6 +431,7 @@ def main():
         torch.cuda.empty_cache()
         out['split_precision'] = split_leg(args)
         out['adjoint'] = adjoint_leg(args)
+        out['streamed_two_class'] = stream_leg(args)
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(fr, size, args.num_rot, grid, args.cpu_frames,
                                            args.cpu_states)

@@ -150,13 +150,15 @@ int rsp_maxlp_destroy(rsp_maxlp* plan);
 int rsp_xterm_gemm(int mode, int64_t M, int64_t N, int64_t K, const void* A, const void* B,
                    void* C, rsp_stream_t stream);
 
-/* Rotation-form plan over the scored pixels (probmask == 0 of emc.py:89-98):
- *   O_r = bilinear rotation of the complex model by the r-th angle (kernels.cu:6-42
+/* Rotation-form plan over the scored pixels (probmask == 0 of emc.py:89-98), with
+ * hypotheses h = c*R + r (class c < C, set by option 3; rotation r < R = num_rot):
+ *   O_h = bilinear rotation of class c's complex model by the r-th angle (kernels.cu:6-42
  *         geometry), R_d = rel_scale * sphere(dia_d) * shift ramp(sx_d, sy_d) (the
- *         reference term of kernels.cu:44-82), W = scale_d * |O_r + R_d|^2,
- *   LL[d*R + r] = sum_photons K log W - sum_pixels W (log clamp of kernels.cu:37-40),
- *   prob[d*R + r] = softmax_r LL (FP64), logz[d] = log sum_r exp LL.
- * rot_cos/rot_sin: host, num_rot (num_rot <= 768). */
+ *         reference term of kernels.cu:44-82), W = scale_d * |O_h + R_d|^2,
+ *   LL[d*H + h] = sum_photons K log W - sum_pixels W (log clamp of kernels.cu:37-40),
+ *   prob[d*H + h] = softmax_h LL (FP64), logz[d] = log sum_h exp LL, H = C*R.
+ * Models are C consecutive c16 npix arrays.  rot_cos/rot_sin: host, num_rot
+ * (C * num_rot <= 1536). */
 typedef struct rsp_rot rsp_rot;
 int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_cos,
                    const double* rot_sin, double rel_scale, rsp_rot** out);
@@ -166,24 +168,32 @@ int rsp_rot_set_frames(rsp_rot* plan, int64_t num_data, const int32_t* ones,
                        const int32_t* multi, const int32_t* place_ones,
                        const int32_t* place_multi, const int32_t* count_multi);
 /* Keys: 1 = cross-term precision (0 FP32 rocBLAS, 1 split BF16 MFMA = default, 2 FP64
- * with an FP64 log term), 2 = frames per chunk (0 = about 8 GB of operand rows),
- * 5 = per-kernel timing (read with rsp_rot_timing). */
+ * with an FP64 log term), 2 = frames per GEMM chunk (0 = about 8 GB of operand rows;
+ * 4096 for the adjoint), 3 = number of object classes C (default 1), 5 = per-kernel
+ * timing (read with rsp_rot_timing).  Refused while a pass is open. */
 int rsp_rot_set_option(rsp_rot* plan, int key, int64_t value);
 /* model_c: device c16 npix; sx, sy, dia, scales: device f64 N; ll, prob: device f64 N*R
  * (either may be NULL); logz: device f64 N (may be NULL).  Synchronises `stream`. */
 int rsp_rot_estep(rsp_rot* plan, const double* model_c, const double* sx, const double* sy,
                   const double* dia, const double* scales, double* ll, double* prob,
                   double* logz, rsp_stream_t stream);
-/* Merged intensity model (device f64 npix) from prob (device f64 N*R): for every
- * rotation merge_all (kernels.cu:139-152) then slice_merge (kernels.cu:84-114), divided
- * by the slice_merge weights times sum_d prob*scale; 0 where the weight is 0.  acc_out
- * and weights_out (device f64 npix) may be NULL.  Synchronises `stream`. */
+/* As rsp_rot_estep, plus best (device i32 N, may be NULL): the first hypothesis index
+ * attaining max_h LL[d, h] (the first-occurrence argmax of emc.py:207-209). */
+int rsp_rot_estep_best(rsp_rot* plan, const double* model_c, const double* sx,
+                       const double* sy, const double* dia, const double* scales, double* ll,
+                       double* prob, double* logz, int32_t* best, rsp_stream_t stream);
+/* Merged intensity models (device f64 C x npix) from prob (device f64 N*H): for every
+ * hypothesis merge_all (kernels.cu:139-152) then slice_merge (kernels.cu:84-114), summed
+ * over the rotations of each class and divided by the slice_merge weights times
+ * sum_d prob*scale; 0 where the weight is 0.  acc_out and weights_out (device f64
+ * C x npix) may be NULL.  Synchronises `stream`. */
 int rsp_rot_merge(rsp_rot* plan, const double* prob, const double* scales, double* model_out,
                   double* acc_out, double* weights_out, rsp_stream_t stream);
-/* Gradient of L = -sum_d log((1/R) sum_r exp LL[d, r]) (uniform rotation prior) at the
- * given inputs, in FP64, recomputing O_r and R_d (nothing of the forward pass is kept but
- * prob).  prob: device f64 N*R, the softmax of LL of rsp_rot_estep at the same inputs.
- * Outputs (device): g_model c16 npix as dL/dRe + i dL/dIm; g_sx, g_sy, g_dia f64 N.
+/* Gradient of L = -sum_d log((1/H) sum_h exp LL[d, h]) (uniform prior over classes and
+ * rotations) at the given inputs, in FP64, recomputing O_h and R_d (nothing of the forward
+ * pass is kept but prob).  prob: device f64 N*H, the softmax of LL of rsp_rot_estep at the
+ * same inputs.  Outputs (device): g_model c16 C x npix as dL/dRe + i dL/dIm; g_sx, g_sy,
+ * g_dia f64 N.
  * dL/dW = -P (K/W - 1) (only -P where W < 1e-20, the log clamp of kernels.cu:37-40),
  * scattered through |O_r + R_d|^2, the sphere/ramp term of kernels.cu:44-82 and the
  * transpose of the bilinear rotation of kernels.cu:6-42 (slice_merge's scatter,
@@ -193,10 +203,43 @@ int rsp_rot_adjoint(rsp_rot* plan, const double* model_c, const double* sx, cons
                     const double* dia, const double* scales, const double* prob,
                     double* g_model, double* g_sx, double* g_sy, double* g_dia,
                     rsp_stream_t stream);
+/* Streamed frames.  rsp_rot_pack (host, synchronous) converts num_data frames of the
+ * dset.py layout into the plan's scored CSR in both orders: fo (num_data+1 i64 offsets),
+ * fj (i32 compact pixel), fk (f32 count) by frame; po (K+1 i64), pd (i32 frame index within
+ * the chunk), pk (f32) by pixel.  With all six outputs NULL it only returns the entry count
+ * in *nnz (sizes: fj/fk/pd/pk nnz, po K+1 with K = rsp_rot_info()[1]).
+ * A pass evaluates one model over chunks of frames whose packed arrays the caller has
+ * copied to the device (for example from pinned host memory on a copy stream):
+ *   rsp_rot_pass_begin(plan, model, flags, max_chunk, stream)   flags: 1 merge,
+ *       2 frame gradients, 4 model gradient (the loss of rsp_rot_adjoint summed over all
+ *       chunks' frames)
+ *   rsp_rot_pass_chunk(plan, n, fo, fj, fk, po, pd, pk, sx, sy, dia, scales, ll, prob,
+ *       logz, best, g_sx, g_sy, g_dia, stream)   all device, chunk-local (n frames);
+ *       po/pd/pk only needed for flags 1 and 4; ll, prob, logz, best may be NULL;
+ *       g_* needed for flag 2.  Work is queued on `stream` (no synchronisation).
+ *   rsp_rot_pass_end(plan, model_out, acc_out, weights_out, g_model, stream)   merged
+ *       intensities (C x npix f64, flag 1) and model gradient (C x npix c16, flag 4);
+ *       synchronises `stream`.
+ * The chunks of a pass must be queued on one stream in order. */
+int rsp_rot_pack(const rsp_rot* plan, int64_t num_data, const int32_t* ones,
+                 const int32_t* multi, const int32_t* place_ones, const int32_t* place_multi,
+                 const int32_t* count_multi, int64_t* fo, int32_t* fj, float* fk, int64_t* po,
+                 int32_t* pd, float* pk, int64_t* nnz);
+int rsp_rot_pass_begin(rsp_rot* plan, const double* model_c, int flags, int64_t max_chunk,
+                       rsp_stream_t stream);
+int rsp_rot_pass_chunk(rsp_rot* plan, int64_t n, const int64_t* fo, const int32_t* fj,
+                       const float* fk, const int64_t* po, const int32_t* pd, const float* pk,
+                       const double* sx, const double* sy, const double* dia,
+                       const double* scales, double* ll, double* prob, double* logz,
+                       int32_t* best, double* g_sx, double* g_sy, double* g_dia,
+                       rsp_stream_t stream);
+int rsp_rot_pass_end(rsp_rot* plan, double* model_out, double* acc_out, double* weights_out,
+                     double* g_model, rsp_stream_t stream);
 /* host double[5]: ms of slices, reference rows, GEMM, log term + LSE; chunks. */
 int rsp_rot_timing(const rsp_rot* plan, double* out);
-/* host int64[7]: frames, scored pixels, padded K, padded rotations, chunk, photon
- * entries in the scored pixels, distinct radii. */
+/* host int64[10]: frames, scored pixels K, padded GEMM K, padded hypotheses (GEMM rows),
+ * chunk, photon entries in the scored pixels, distinct radii, classes C, hypotheses H,
+ * hypothesis stride of the pixel-major arrays. */
 int rsp_rot_info(const rsp_rot* plan, int64_t* out);
 int rsp_rot_destroy(rsp_rot* plan);
 

@@ -37,9 +37,10 @@ def scored_pixels(size):
     return np.nonzero(probmask(size) == 0)[0]
 
 
-def rot_cos_sin(num_rot):
-    """cos/sin of j*pi/num_rot from the C library (emc.py:192)."""
-    a = [j * PI / num_rot for j in range(num_rot)]
+def rot_cos_sin(num_rot, full_turn=False):
+    """cos/sin of j*pi/num_rot from the C library (emc.py:192), or of j*2pi/num_rot."""
+    step = (2. if full_turn else 1.) * PI / num_rot
+    a = [j * step for j in range(num_rot)]
     return np.array([math.cos(v) for v in a]), np.array([math.sin(v) for v in a])
 
 
@@ -247,3 +248,51 @@ def loss_grad(model_c, size, num_rot, K, sx, sy, dia, scales=None, rel_scale=1.,
     red = lambda dR: (c * (Z.real * dR.real[:, None, :] +
                            Z.imag * dR.imag[:, None, :])).sum((1, 2))
     return total_loss(LL), g_model, red(dsx), red(dsy), red(ddia)
+
+
+# ----------------------------------------------------------------------------------
+# several object classes: hypotheses h = c * R + r (class-major), one model per class
+# ----------------------------------------------------------------------------------
+
+def rot_loglik_multi(models, size, num_rot, K, sx, sy, dia, scales=None, rel_scale=1.,
+                     pix=None, cos_sin=None):
+    """LL[d, c*R + r] for the class models ``models`` (C, npix): each class block is
+    :func:`rot_loglik` of that class's model (the reference term is shared)."""
+    return np.concatenate([rot_loglik(m, size, num_rot, K, sx, sy, dia, scales, rel_scale, pix,
+                                      cos_sin)[0] for m in np.atleast_2d(models)], axis=1)
+
+
+def merge_model_multi(P, K, scales, size, num_rot, pix=None, cos_sin=None):
+    """Per-class merged intensities: class c uses P[:, c*R:(c+1)*R] in :func:`merge_model`
+    (its weights w_r = sum_d P[d, c*R + r] * scale_d).  Returns (C, npix) arrays."""
+    C = P.shape[1] // num_rot
+    out = [merge_model(P[:, c * num_rot:(c + 1) * num_rot], K, scales, size, num_rot, pix,
+                       cos_sin) for c in range(C)]
+    return tuple(np.stack(v) for v in zip(*out))
+
+
+def loss_grad_multi(models, size, num_rot, K, sx, sy, dia, scales=None, rel_scale=1., pix=None,
+                    cos_sin=None):
+    """Loss -sum_d log((1/(C R)) sum_h exp LL[d, h]) and its FP64 gradient for several
+    classes: (loss, g_models (C, npix) complex, g_sx, g_sy, g_dia)."""
+    models = np.atleast_2d(models)
+    pix = scored_pixels(size) if pix is None else pix
+    N = K.shape[0]
+    scales = np.ones(N) if scales is None else np.asarray(scales, dtype='f8')
+    pieces = [rot_loglik(m, size, num_rot, K, sx, sy, dia, scales, rel_scale, pix, cos_sin)
+              for m in models]
+    LL = np.concatenate([p[0] for p in pieces], axis=1)
+    P, _ = lse(LL)
+    dsx, dsy, ddia = ref_derivs(size, pix, sx, sy, dia, rel_scale)
+    g_models = []
+    g = np.zeros((3, N))
+    for c, (_, pc) in enumerate(pieces):
+        Pc = P[:, c * num_rot:(c + 1) * num_rot]
+        W, Z = pc['W'], pc['Z']
+        ok = W >= 1.e-20
+        dW = np.where(ok, K[:, None, :] / np.where(ok, W, 1.), 0.) - 1.
+        cc = -Pc[:, :, None] * dW * (2. * scales[:, None, None])
+        g_models.append(interp_transpose((cc * Z).sum(0), size, num_rot, pix, cos_sin))
+        for i, dR in enumerate((dsx, dsy, ddia)):
+            g[i] += (cc * (Z.real * dR.real[:, None, :] + Z.imag * dR.imag[:, None, :])).sum((1, 2))
+    return total_loss(LL), np.stack(g_models), g[0], g[1], g[2]

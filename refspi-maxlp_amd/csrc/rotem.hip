@@ -1,32 +1,39 @@
-// Rotation-form E-step and merge on one GPU.
+// Rotation-form E-step, merge and adjoint on one GPU.
 //
-// Model: per-frame reference parameters (shift sx_d, sy_d, diameter dia_d) and in-plane
-// rotation r = j*pi/R as the latent variable.  Over the scored pixels (probmask == 0,
-// emc.py:89-98), compact index j:
-//   O_r(j)  = bilinear rotation of the complex model (kernels.cu:6-42 geometry, re and
-//             im separately, out of bounds -> 0)
+// Model: per-frame reference parameters (shift sx_d, sy_d, diameter dia_d) and a latent
+// hypothesis h = c * R + r: object class c < C (one complex model per class) and in-plane
+// rotation r = j*pi/R.  Over the scored pixels (probmask == 0, emc.py:89-98), compact
+// index j:
+//   O_h(j)  = bilinear rotation of class c's complex model by r (kernels.cu:6-42
+//             geometry, re and im separately, out of bounds -> 0)
 //   R_d(j)  = rel * S(pi*|p-c|*dia_d/size) * exp(2*pi*i*((x-c)*sx_d + (y-c)*sy_d)/size)
 //             (the sphere term of kernels.cu:44-82, s == 0 -> 1e-5)
-//   W_dr(j) = scale_d * |O_r(j) + R_d(j)|^2
-//   LL_dr   = sum_{photons} K log W_dr (log clamp of kernels.cu:37-40)
-//             - scale_d * (|O_r|^2 + |R_d|^2 + 2 X[d, r]),   X = Re sum_j O_r conj(R_d)
+//   W_dh(j) = scale_d * |O_h(j) + R_d(j)|^2
+//   LL_dh   = sum_{photons} K log W_dh (log clamp of kernels.cu:37-40)
+//             - scale_d * (|O_h|^2 + |R_d|^2 + 2 X[d, h]),   X = Re sum_j O_h conj(R_d)
 // X is the cross-term GEMM (xterm.hip) in FP32 (rocBLAS), split BF16 (MFMA) or FP64
-// (rocBLAS).  P(r|d) and log Z_d come from an FP64 log-sum-exp over r.  The merge is
-// merge_all + slice_merge (kernels.cu:139-152, 84-114) for every rotation, normalised
-// by the slice_merge weights times sum_d P(r|d) scale_d.
+// (rocBLAS).  P(h|d) and log Z_d come from an FP64 log-sum-exp over h.  The merge is
+// merge_all + slice_merge (kernels.cu:139-152, 84-114) for every hypothesis, normalised
+// per class by the slice_merge weights times sum_d P(h|d) scale_d.
 //
 // Kernels
-//   k_rot_slices   O_r(j) for a 64-pixel x 32-rotation tile: GEMM operand rows (A), the
-//                  pixel-major copy OT[j][r] through an LDS transpose, per-tile |O|^2.
+//   k_rot_slices   O_h(j) for a 64-pixel x 32-hypothesis tile: GEMM operand rows (A), the
+//                  pixel-major copy OT[j][h] through an LDS transpose, per-tile |O_h|^2.
 //   k_ref_rows     one work-group per frame: S over the distinct radii in LDS, separable
 //                  phase ramps, GEMM operand rows (B) and |R_d|^2.
-//   k_rot_ll       one wave per frame, lanes over rotations: sparse log term over the
-//                  frame's photons (OT rows are coalesced 64-rotation reads), the dense
-//                  term, and the FP64 log-sum-exp.
-//   k_merge_view   one work-group per pixel (pixel-major photon lists): VT[j][r] =
-//                  sum_d K P(r|d), no atomics.
-//   k_merge_model  one thread per model pixel: slice_merge in gather form (the 3 x 3
-//                  view pixels around the inverse-rotated position), no atomics.
+//   k_rot_ll       one wave per frame, lanes over hypotheses: sparse log term over the
+//                  frame's photons (OT rows are coalesced 64-hypothesis reads), the dense
+//                  term, the FP64 log-sum-exp and the first-occurrence argmax.
+//   k_merge_view   one work-group per pixel (pixel-major photon lists): VT[j][h] +=
+//                  sum_d K P(h|d), no atomics.
+//   k_merge_model  one thread per (model pixel, class): slice_merge in gather form (the
+//                  3 x 3 view pixels around the inverse-rotated position), no atomics.
+//   k_adj_*        the adjoint (below).
+//
+// Frames come either resident (rsp_rot_set_frames) or as chunks of the caller's (packed by
+// rsp_rot_pack, e.g. into pinned host memory and copied in on a side stream) through the
+// pass API: rsp_rot_pass_begin / rsp_rot_pass_chunk / rsp_rot_pass_end, which accumulates
+// the merge and the model gradient over chunks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,8 +55,19 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 constexpr double kPi = 3.14159265358979323846;
 
 enum Mode { kFP32 = 0, kSplit = 1, kFP64 = 2 };
+enum PassFlags { kPassMerge = 1, kPassFrameGrad = 2, kPassModelGrad = 4 };
 
-// O_r(j) of the bilinear rotation, weights formed first as the oracle does.
+// Lane-iteration counts of the hypothesis kernels; the hypothesis stride is 64 * NT.
+constexpr int kNTs[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 20, 24};
+constexpr int kMaxH = 64 * 24;
+
+int nt_for(int64_t H) {
+  for (int nt : kNTs)
+    if (64 * (int64_t)nt >= H) return nt;
+  return 0;
+}
+
+// O_h(j) of the bilinear rotation, weights formed first as the oracle does.
 __device__ __forceinline__ void rot_sample(const double* __restrict__ m, int size, int t,
                                            double ac, double as, double& vr, double& vi) {
   const int cen = size / 2;
@@ -73,23 +91,28 @@ __device__ __forceinline__ void rot_sample(const double* __restrict__ m, int siz
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_rot_slices(
-    const double* __restrict__ model_c, int size, int K, int R, const int32_t* __restrict__ scored,
-    const double* __restrict__ rcos, const double* __restrict__ rsin, void* __restrict__ A,
-    int64_t Mpad, int64_t Kp, void* __restrict__ OT, int Rs, double* __restrict__ norm_part) {
+    const double* __restrict__ model_c, int size, int K, int R, int H,
+    const int32_t* __restrict__ scored, const double* __restrict__ rcos,
+    const double* __restrict__ rsin, void* __restrict__ A, int64_t Mpad, int64_t Kp,
+    void* __restrict__ OT, int Hs, double* __restrict__ norm_part) {
   __shared__ double2 tile[32][65];
   const int tid = threadIdx.x;
   const int jl = tid & 63, rg = tid >> 6;
   const int j = blockIdx.x * 64 + jl;
-  const int r0 = blockIdx.y * 32;
+  const int h0 = blockIdx.y * 32;
   const int t = j < K ? scored[j] : 0;
+  const int64_t npix2 = 2 * (int64_t)size * size;
 #pragma unroll
   for (int u = 0; u < 8; ++u) {
-    const int rl = rg + 4 * u, r = r0 + rl;
+    const int hl = rg + 4 * u, h = h0 + hl;
     double vr = 0., vi = 0.;
-    if (j < K && r < R) rot_sample(model_c, size, t, rcos[r], rsin[r], vr, vi);
-    tile[rl][jl] = make_double2(vr, vi);
-    if (j < K && r < R) {
-      const int64_t k = (int64_t)r * Kp + 2 * j;
+    if (j < K && h < H) {
+      const int c = h / R, r = h - c * R;
+      rot_sample(model_c + c * npix2, size, t, rcos[r], rsin[r], vr, vi);
+    }
+    tile[hl][jl] = make_double2(vr, vi);
+    if (j < K && h < H) {
+      const int64_t k = (int64_t)h * Kp + 2 * j;
       if constexpr (MODE == kFP32) {
         *reinterpret_cast<float2*>((float*)A + k) = make_float2((float)vr, (float)vi);
       } else if constexpr (MODE == kFP64) {
@@ -99,43 +122,44 @@ __global__ __launch_bounds__(256) void k_rot_slices(
         __bf16* al = ah + Mpad * Kp;
         const float fr = (float)vr, fi = (float)vi;
         const __bf16 hr = (__bf16)fr, hi = (__bf16)fi;
-        bf16x2 h = {hr, hi};
-        bf16x2 l = {(__bf16)(fr - (float)hr), (__bf16)(fi - (float)hi)};
-        *reinterpret_cast<bf16x2*>(ah + k) = h;
-        *reinterpret_cast<bf16x2*>(al + k) = l;
+        bf16x2 hv = {hr, hi};
+        bf16x2 lv = {(__bf16)(fr - (float)hr), (__bf16)(fi - (float)hi)};
+        *reinterpret_cast<bf16x2*>(ah + k) = hv;
+        *reinterpret_cast<bf16x2*>(al + k) = lv;
       }
     }
   }
   __syncthreads();
-  // per-rotation partial |O|^2 over this pixel tile (FP64, fixed order)
+  // per-hypothesis partial |O|^2 over this pixel tile (FP64, fixed order)
   if (tid < 32) {
     double s = 0.;
     for (int q = 0; q < 64; ++q) {
       const double2 v = tile[tid][q];
       s += v.x * v.x + v.y * v.y;
     }
-    if (r0 + tid < R) norm_part[(int64_t)blockIdx.x * R + r0 + tid] = s;
+    if (h0 + tid < H) norm_part[(int64_t)blockIdx.x * H + h0 + tid] = s;
   }
-  // pixel-major copy: 32 consecutive rotations per pixel row
-  const int rl = tid & 31;
+  // pixel-major copy: 32 consecutive hypotheses per pixel row
+  const int hl = tid & 31;
   for (int q = tid >> 5; q < 64; q += 8) {
-    const int jj = blockIdx.x * 64 + q, r = r0 + rl;
-    if (jj >= K || r >= Rs) continue;
-    const double2 v = tile[rl][q];
+    const int jj = blockIdx.x * 64 + q, h = h0 + hl;
+    if (jj >= K || h >= Hs) continue;
+    const double2 v = tile[hl][q];
     if constexpr (MODE == kFP64)
-      ((double2*)OT)[(int64_t)jj * Rs + r] = v;
+      ((double2*)OT)[(int64_t)jj * Hs + h] = v;
     else
-      ((float2*)OT)[(int64_t)jj * Rs + r] = make_float2((float)v.x, (float)v.y);
+      ((float2*)OT)[(int64_t)jj * Hs + h] = make_float2((float)v.x, (float)v.y);
   }
 }
 
-__global__ void k_norm_reduce(const double* __restrict__ part, int ntiles, int R,
-                              double* __restrict__ out) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R) return;
+// out[i] (+)= sum_q part[q * n + i] in a fixed order.
+__global__ void k_norm_reduce(const double* __restrict__ part, int ntiles, int n,
+                              double* __restrict__ out, int accumulate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   double s = 0.;
-  for (int q = 0; q < ntiles; ++q) s += part[(int64_t)q * R + r];
-  out[r] = s;
+  for (int q = 0; q < ntiles; ++q) s += part[(int64_t)q * n + i];
+  out[i] = accumulate ? out[i] + s : s;
 }
 
 __device__ __forceinline__ double sphere_guarded(double s) {
@@ -188,10 +212,10 @@ __global__ __launch_bounds__(256) void k_ref_rows(
       __bf16* bl = bh + Bpad * Kp;
       const float fr = (float)vr, fi = (float)vi;
       const __bf16 hr = (__bf16)fr, hi = (__bf16)fi;
-      bf16x2 h = {hr, hi};
-      bf16x2 l = {(__bf16)(fr - (float)hr), (__bf16)(fi - (float)hi)};
-      *reinterpret_cast<bf16x2*>(bh + k) = h;
-      *reinterpret_cast<bf16x2*>(bl + k) = l;
+      bf16x2 hv = {hr, hi};
+      bf16x2 lv = {(__bf16)(fr - (float)hr), (__bf16)(fi - (float)hi)};
+      *reinterpret_cast<bf16x2*>(bh + k) = hv;
+      *reinterpret_cast<bf16x2*>(bl + k) = lv;
     }
   }
   red[threadIdx.x] = acc;
@@ -234,17 +258,17 @@ __device__ __forceinline__ T log_floor(T w) {
     return w >= 1.e-20 ? log(w) : -1000.;
 }
 
-// One wave per frame of [d0, d1): LL, P, log Z.  NT = ceil(Rs / 64) lanes-iterations.
+// One wave per frame of [d0, d1): LL, P, log Z, argmax.  Hypotheses h = lane + 64 t.
 template <typename T, int NT>
 __global__ __launch_bounds__(256) void k_rot_ll(
-    int size, int R, int Rs, double rel, const int64_t* __restrict__ fo,
+    int size, int H, int Hs, double rel, const int64_t* __restrict__ fo,
     const int32_t* __restrict__ fj, const float* __restrict__ fk,
     const int32_t* __restrict__ scored, const void* __restrict__ OT_,
     const double* __restrict__ normO, const double* __restrict__ normR,
     const void* __restrict__ X_, int64_t ldx, const double* __restrict__ sx,
     const double* __restrict__ sy, const double* __restrict__ dia,
     const double* __restrict__ scales, int64_t d0, int64_t d1, double* __restrict__ ll,
-    double* __restrict__ prob, double* __restrict__ logz) {
+    double* __restrict__ prob, double* __restrict__ logz, int32_t* __restrict__ best) {
   typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
   __shared__ int32_t s_j[4][64];
   __shared__ T s_k[4][64], s_rr[4][64], s_ri[4][64];
@@ -273,7 +297,7 @@ __global__ __launch_bounds__(256) void k_rot_ll(
     __builtin_amdgcn_wave_barrier();
     const int n = (int)min((int64_t)64, o1 - base);
     for (int q = 0; q < n; ++q) {
-      const T2* row = OT + (int64_t)s_j[w][q] * Rs + lane;
+      const T2* row = OT + (int64_t)s_j[w][q] * Hs + lane;
       const T kq = s_k[w][q], rr = s_rr[w][q], ri = s_ri[w][q];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
@@ -288,70 +312,113 @@ __global__ __launch_bounds__(256) void k_rot_ll(
   const double nr = normR[d - d0];
   const T* X = (const T*)X_;
   double m = -INFINITY;
+  int hbest = 0x7fffffff;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int r = lane + 64 * t;
-    if (r < R) {
-      acc[t] -= scale * (normO[r] + nr + 2. * (double)X[(d - d0) * ldx + r]);
-      m = fmax(m, acc[t]);
+    const int h = lane + 64 * t;
+    if (h < H) {
+      acc[t] -= scale * (normO[h] + nr + 2. * (double)X[(d - d0) * ldx + h]);
+      if (acc[t] > m) {           // first occurrence within the lane (h increases with t)
+        m = acc[t];
+        hbest = h;
+      }
     }
   }
+  const double lm = m;
   m = wave_max(m);
+  // smallest hypothesis index attaining the wave maximum
+  int hb = (lm == m) ? hbest : 0x7fffffff;
+  for (int o = 32; o > 0; o >>= 1) hb = min(hb, __shfl_xor(hb, o));
   double s = 0.;
 #pragma unroll
   for (int t = 0; t < NT; ++t)
-    if (lane + 64 * t < R) s += exp(acc[t] - m);
+    if (lane + 64 * t < H) s += exp(acc[t] - m);
   s = wave_sum(s);
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int r = lane + 64 * t;
-    if (r >= R) continue;
-    if (ll) ll[d * R + r] = acc[t];
-    if (prob) prob[d * R + r] = exp(acc[t] - m) / s;
+    const int h = lane + 64 * t;
+    if (h >= H) continue;
+    if (ll) ll[d * H + h] = acc[t];
+    if (prob) prob[d * H + h] = exp(acc[t] - m) / s;
   }
-  if (lane == 0 && logz) logz[d] = m + log(s);
+  if (lane == 0) {
+    if (logz) logz[d] = m + log(s);
+    if (best) best[d] = hb;
+  }
 }
 
-// wsum partials: part[b][r] = sum_{d in block b} P[d, r] * scale_d
+// wsum partials: part[b][h] = sum_{d in block b} P[d, h] * scale_d
 __global__ void k_wsum_part(const double* __restrict__ P, const double* __restrict__ scales,
-                            int64_t N, int R, int64_t per, double* __restrict__ part) {
-  const int r = blockIdx.y * blockDim.x + threadIdx.x;
-  if (r >= R) return;
+                            int64_t N, int H, int64_t per, double* __restrict__ part) {
+  const int h = blockIdx.y * blockDim.x + threadIdx.x;
+  if (h >= H) return;
   const int64_t a = (int64_t)blockIdx.x * per, b = min(N, a + per);
   double s = 0.;
-  for (int64_t d = a; d < b; ++d) s += P[d * R + r] * scales[d];
-  part[(int64_t)blockIdx.x * R + r] = s;
+  for (int64_t d = a; d < b; ++d) s += P[d * H + h] * scales[d];
+  part[(int64_t)blockIdx.x * H + h] = s;
 }
 
-// VT[j][r] = sum over the photons of pixel j of K * P[d, r] (pixel-major lists).
+// VT[j][h] (+)= sum over the photons of pixel j of K * P[d, h] (pixel-major lists).
 __global__ __launch_bounds__(256) void k_merge_view(
     const int64_t* __restrict__ po, const int32_t* __restrict__ pd,
-    const float* __restrict__ pk, const double* __restrict__ P, int R, int Rs,
-    double* __restrict__ VT) {
+    const float* __restrict__ pk, const double* __restrict__ P, int H, int Hs,
+    double* __restrict__ VT, int accumulate) {
   const int j = blockIdx.x;
   const int64_t e0 = po[j], e1 = po[j + 1];
-  for (int r0 = 0; r0 < Rs; r0 += 256 * 4) {
+  for (int h0 = 0; h0 < Hs; h0 += 256 * 4) {
     double acc[4] = {0., 0., 0., 0.};
     for (int64_t e = e0; e < e1; ++e) {
       const int64_t d = pd[e];
       const double k = (double)pk[e];
-      const double* row = P + d * R;
+      const double* row = P + d * H;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int r = r0 + threadIdx.x + 256 * u;
-        if (r < R) acc[u] += k * row[r];
+        const int h = h0 + threadIdx.x + 256 * u;
+        if (h < H) acc[u] += k * row[h];
       }
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int r = r0 + threadIdx.x + 256 * u;
-      if (r < Rs) VT[(int64_t)j * Rs + r] = r < R ? acc[u] : 0.;
+      const int h = h0 + threadIdx.x + 256 * u;
+      if (h >= Hs) continue;
+      double* v = VT + (int64_t)j * Hs + h;
+      const double a = h < H ? acc[u] : 0.;
+      *v = accumulate ? *v + a : a;
     }
   }
 }
 
-// slice_merge of every rotation in gather form; one thread per model pixel m.
-__global__ void k_merge_model(int size, int R, int Rs, const double* __restrict__ rcos,
+// Visit the scored view pixels j whose bilinear sample under rotation r touches model
+// pixel (mx, my), with the bilinear weight w: f(j, w).
+template <typename F>
+__device__ __forceinline__ void for_each_source(int size, int mx, int my, double ac, double as,
+                                                const int32_t* __restrict__ compact_of, F f) {
+  const int cen = size / 2;
+  // inverse rotation of m: the view pixels that can reach m lie within +-1 of it
+  const double mxc = (double)(mx - cen), myc = (double)(my - cen);
+  const int x0 = (int)rint(mxc * ac + myc * as + cen);
+  const int y0 = (int)rint(-mxc * as + myc * ac + cen);
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int x = x0 + dx, y = y0 + dy;
+      if (x < 0 || x >= size || y < 0 || y >= size) continue;
+      const int j = compact_of[x * size + y];
+      if (j < 0) continue;
+      double tx, ty;
+      rot_source(x, y, cen, ac, as, tx, ty);
+      const int ix = (int)floor(tx), iy = (int)floor(ty);
+      if (ix < 0 || ix > size - 2 || iy < 0 || iy > size - 2) continue;
+      const int ox = mx - ix, oy = my - iy;
+      if (ox < 0 || ox > 1 || oy < 0 || oy > 1) continue;
+      const double fx = tx - ix, fy = ty - iy;
+      const double cx = 1. - fx, cy = 1. - fy;
+      f(j, (ox ? fx : cx) * (oy ? fy : cy));
+    }
+}
+
+// slice_merge of every rotation of class c = blockIdx.y in gather form; one thread per
+// model pixel m.  Outputs at c * npix + m.
+__global__ void k_merge_model(int size, int R, int Hs, const double* __restrict__ rcos,
                               const double* __restrict__ rsin,
                               const int32_t* __restrict__ compact_of,
                               const double* __restrict__ VT, const double* __restrict__ wsum,
@@ -360,59 +427,43 @@ __global__ void k_merge_model(int size, int R, int Rs, const double* __restrict_
   const int npix = size * size;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= npix) return;
-  const int mx = m / size, my = m - (m / size) * size, cen = size / 2;
+  const int c = blockIdx.y;
+  const int mx = m / size, my = m - (m / size) * size;
   double acc = 0., wts = 0.;
   for (int r = 0; r < R; ++r) {
-    const double ac = rcos[r], as = rsin[r];
-    // inverse rotation of m: the view pixels that can reach m lie within +-1 of it
-    const double mxc = (double)(mx - cen), myc = (double)(my - cen);
-    const int x0 = (int)rint(mxc * ac + myc * as + cen);
-    const int y0 = (int)rint(-mxc * as + myc * ac + cen);
+    const int h = c * R + r;
     double wr = 0.;
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int x = x0 + dx, y = y0 + dy;
-        if (x < 0 || x >= size || y < 0 || y >= size) continue;
-        const int j = compact_of[x * size + y];
-        if (j < 0) continue;
-        double tx, ty;
-        rot_source(x, y, cen, ac, as, tx, ty);
-        const int ix = (int)floor(tx), iy = (int)floor(ty);
-        if (ix < 0 || ix > size - 2 || iy < 0 || iy > size - 2) continue;
-        const int ox = mx - ix, oy = my - iy;
-        if (ox < 0 || ox > 1 || oy < 0 || oy > 1) continue;
-        const double fx = tx - ix, fy = ty - iy;
-        const double cx = 1. - fx, cy = 1. - fy;
-        const double w = (ox ? fx : cx) * (oy ? fy : cy);
-        acc += VT[(int64_t)j * Rs + r] * w;
-        wr += w;
-      }
-    wts += wr * wsum[r];
+    for_each_source(size, mx, my, rcos[r], rsin[r], compact_of, [&](int j, double w) {
+      acc += VT[(int64_t)j * Hs + h] * w;
+      wr += w;
+    });
+    wts += wr * wsum[h];
   }
-  if (acc_out) acc_out[m] = acc;
-  if (wts_out) wts_out[m] = wts;
-  model[m] = wts > 0. ? acc / wts : 0.;
+  const int64_t o = (int64_t)c * npix + m;
+  if (acc_out) acc_out[o] = acc;
+  if (wts_out) wts_out[o] = wts;
+  model[o] = wts > 0. ? acc / wts : 0.;
 }
 
 // ---------------------------------------------------------------------------------
-// Adjoint of L = -sum_d log((1/R) sum_r exp LL_dr) (recomputes O_r and R_d; FP64).
-//   dL/dW_drj = -P_dr (K_dj / W_drj - 1)        (-P_dr * -1 where W < 1e-20: log clamp)
-//   dW/dtheta = 2 scale_d Re(conj(Z) dZ/dtheta),   Z = O_r(j) + R_d(j)
-// Model:  g_O_r(j) = -2 sum_{d in photons(j)} P_dr K Z / |Z|^2 + 2 ws_r O_r(j)
-//                    + 2 sum_d P_dr scale_d R_d(j)          (ws_r = sum_d P_dr scale_d)
-//         g_model  = sum_r Interp_r^T g_O_r               (slice_merge's transpose form)
-// Frame:  g_theta_d = -2 sum_r P_dr sum_photons K Re(conj(Z) dR) / |Z|^2
+// Adjoint of L = -sum_d log((1/H) sum_h exp LL_dh) (recomputes O_h and R_d; FP64).
+//   dL/dW_dhj = -P_dh (K_dj / W_dhj - 1)        (-P_dh * -1 where W < 1e-20: log clamp)
+//   dW/dtheta = 2 scale_d Re(conj(Z) dZ/dtheta),   Z = O_h(j) + R_d(j)
+// Model:  g_O_h(j) = -2 sum_{d in photons(j)} P_dh K Z / |Z|^2 + 2 ws_h O_h(j)
+//                    + 2 sum_d P_dh scale_d R_d(j)          (ws_h = sum_d P_dh scale_d)
+//         g_model_c = sum_r Interp_r^T g_O_{cR+r}          (slice_merge's transpose form)
+// Frame:  g_theta_d = -2 sum_h P_dh sum_photons K Re(conj(Z) dR) / |Z|^2
 //                    + 2 scale_d sum_j Re(conj(Obar_d(j) + R_d(j)) dR_d(j)),
-//         Obar_d = sum_r P_dr O_r                          (sum_r P_dr = 1)
+//         Obar_d = sum_h P_dh O_h                          (sum_h P_dh = 1)
 // ---------------------------------------------------------------------------------
 
-// Q[dl * R + r] = P[(d0 + dl) * R + r] * scale[d0 + dl]
+// Q[dl * H + h] = P[(d0 + dl) * H + h] * scale[d0 + dl]
 __global__ void k_scaled_prob(const double* __restrict__ P, const double* __restrict__ scales,
-                              int64_t d0, int64_t n, int R, double* __restrict__ Q) {
+                              int64_t d0, int64_t n, int H, double* __restrict__ Q) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n * R) return;
-  const int64_t dl = i / R;
-  Q[i] = P[d0 * R + i] * scales[d0 + dl];
+  if (i >= n * H) return;
+  const int64_t dl = i / H;
+  Q[i] = P[d0 * H + i] * scales[d0 + dl];
 }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -484,11 +535,11 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
 }
 
 // One work-group per frame of [d0, d0 + gridDim.x): frame gradients (sx, sy, dia).
-// Dense part: threads over the scored pixels (Obar row of the chunk).  Sparse part: wave w
-// takes photons w, w+4, ...; lanes over rotations r = lane + 64 t.
+// Dense part: threads over the scored pixels (Obar row dl of the sub-chunk).  Sparse part:
+// wave w takes photons w, w+4, ...; lanes over hypotheses h = lane + 64 t.
 template <int NT>
 __global__ __launch_bounds__(256) void k_adj_frame(
-    int size, int K, int R, int Rs, double rel, const int32_t* __restrict__ scored,
+    int size, int K, int H, int Hs, double rel, const int32_t* __restrict__ scored,
     const int32_t* __restrict__ jrad, const double* __restrict__ rad, int nrad,
     const int64_t* __restrict__ fo, const int32_t* __restrict__ fj,
     const float* __restrict__ fk, const double2* __restrict__ OT, const double* __restrict__ P,
@@ -515,7 +566,7 @@ __global__ __launch_bounds__(256) void k_adj_frame(
     ey_ += zr * Dy.x + zi * Dy.y;
     ed_ += zr * Dd.x + zi * Dd.y;
   }
-  // sparse: per rotation sum_photons K Re(conj(Z) dR) / |Z|^2
+  // sparse: per hypothesis sum_photons K Re(conj(Z) dR) / |Z|^2
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double ax[NT], ay[NT], ad[NT];
 #pragma unroll
@@ -526,7 +577,7 @@ __global__ __launch_bounds__(256) void k_adj_frame(
     const double k = (double)fk[q];
     double2 Rv, Dx, Dy, Dd;
     ref_and_derivs(tb, scored[j], jrad[j], size, Rv, Dx, Dy, Dd);
-    const double2* row = OT + (int64_t)j * Rs + lane;
+    const double2* row = OT + (int64_t)j * Hs + lane;
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const double2 o = row[64 * t];
@@ -543,9 +594,9 @@ __global__ __launch_bounds__(256) void k_adj_frame(
   double px = 0., py = 0., pd = 0.;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int r = lane + 64 * t;
-    if (r < R) {
-      const double pr = P[d * R + r];
+    const int h = lane + 64 * t;
+    if (h < H) {
+      const double pr = P[d * H + h];
       px += pr * ax[t];
       py += pr * ay[t];
       pd += pr * ad[t];
@@ -561,18 +612,16 @@ __global__ __launch_bounds__(256) void k_adj_frame(
   }
 }
 
-// One work-group per scored pixel j: g_O_r(j) for every rotation (threads over r), from
-// the pixel-major photon lists, O_r(j) (OT), the dense GEMM term GT (column-major R x Kp)
-// and ws_r.  GO[j * Rs + r] (complex); rotations r >= R get 0.
+// One work-group per scored pixel j: GS[j * Hs + h] (+)= sum over the pixel's photons
+// (d, K) of K P_dh Z / |Z|^2 (threads over hypotheses).  Pixel-major lists po/pd/pk.
 template <int NU>
-__global__ __launch_bounds__(256) void k_adj_pixel(
-    int size, int R, int Rs, double rel, const int32_t* __restrict__ scored,
+__global__ __launch_bounds__(256) void k_adj_pixel_acc(
+    int size, int H, int Hs, double rel, const int32_t* __restrict__ scored,
     const int64_t* __restrict__ po, const int32_t* __restrict__ pd,
     const float* __restrict__ pk, const double2* __restrict__ OT, const double* __restrict__ P,
-    const double* __restrict__ GT, const double* __restrict__ wsum,
     const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ dia, const double* __restrict__ scales,
-    double2* __restrict__ GO) {
+    double2* __restrict__ GS, int accumulate) {
   __shared__ double s_rr[256], s_ri[256], s_k[256], s_sc[256];
   __shared__ int32_t s_d[256];
   const int j = blockIdx.x, tid = threadIdx.x;
@@ -580,8 +629,8 @@ __global__ __launch_bounds__(256) void k_adj_pixel(
   double2 O[NU], acc[NU];
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
-    const int r = tid + 256 * u;
-    O[u] = r < Rs ? OT[(int64_t)j * Rs + r] : make_double2(0., 0.);
+    const int h = tid + 256 * u;
+    O[u] = h < Hs ? OT[(int64_t)j * Hs + h] : make_double2(0., 0.);
     acc[u] = make_double2(0., 0.);
   }
   const int64_t e0 = po[j], e1 = po[j + 1];
@@ -601,15 +650,15 @@ __global__ __launch_bounds__(256) void k_adj_pixel(
     const int n = (int)min((int64_t)256, e1 - base);
     for (int q = 0; q < n; ++q) {
       const double rr = s_rr[q], ri = s_ri[q], k = s_k[q], sc = s_sc[q];
-      const double* prow = P + (int64_t)s_d[q] * R;
+      const double* prow = P + (int64_t)s_d[q] * H;
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
-        const int r = tid + 256 * u;
-        if (r < R) {
+        const int h = tid + 256 * u;
+        if (h < H) {
           const double zr = O[u].x + rr, zi = O[u].y + ri;
           const double n2 = zr * zr + zi * zi;
           if (sc * n2 >= 1.e-20) {
-            const double f = k * prow[r] / n2;
+            const double f = k * prow[h] / n2;
             acc[u].x += f * zr;
             acc[u].y += f * zi;
           }
@@ -620,56 +669,56 @@ __global__ __launch_bounds__(256) void k_adj_pixel(
   }
 #pragma unroll
   for (int u = 0; u < NU; ++u) {
-    const int r = tid + 256 * u;
-    if (r >= Rs) continue;
-    double2 g = make_double2(0., 0.);
-    if (r < R) {
-      const double ws = wsum[r];
-      g.x = -2. * acc[u].x + 2. * ws * O[u].x + 2. * GT[(int64_t)(2 * j) * R + r];
-      g.y = -2. * acc[u].y + 2. * ws * O[u].y + 2. * GT[(int64_t)(2 * j + 1) * R + r];
-    }
-    GO[(int64_t)j * Rs + r] = g;
+    const int h = tid + 256 * u;
+    if (h >= Hs) continue;
+    double2* g = GS + (int64_t)j * Hs + h;
+    const double2 a = h < H ? acc[u] : make_double2(0., 0.);
+    *g = accumulate ? make_double2(g->x + a.x, g->y + a.y) : a;
   }
 }
 
-// g_model(m) = sum_r sum_{scored view pixels j whose bilinear sample touches m} w * GO[j][r]
-// (the transpose of k_rot_slices' interpolation; gather form, one thread per model pixel).
-__global__ void k_adj_model(int size, int R, int Rs, const double* __restrict__ rcos,
+// In place: G[j][h] = -2 GS + 2 ws_h O_h(j) + 2 GT(h, j) (GT column-major H x Kp, re/im at
+// columns 2j, 2j+1); 0 for h >= H.  One thread per (j, h).
+__global__ void k_adj_combine(int K, int H, int Hs, const double2* __restrict__ OT,
+                              const double* __restrict__ GT, const double* __restrict__ wsum,
+                              double2* __restrict__ G) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)K * Hs) return;
+  const int64_t j = i / Hs;
+  const int h = (int)(i - j * Hs);
+  double2 g = make_double2(0., 0.);
+  if (h < H) {
+    const double2 s = G[i], o = OT[i];
+    const double ws = wsum[h];
+    g.x = -2. * s.x + 2. * ws * o.x + 2. * GT[(2 * j) * H + h];
+    g.y = -2. * s.y + 2. * ws * o.y + 2. * GT[(2 * j + 1) * H + h];
+  }
+  G[i] = g;
+}
+
+// g_model[c](m) = sum_r sum_{scored view pixels j whose bilinear sample touches m} w *
+// GO[j][c R + r] (the transpose of k_rot_slices' interpolation; gather form).
+__global__ void k_adj_model(int size, int R, int Hs, const double* __restrict__ rcos,
                             const double* __restrict__ rsin,
                             const int32_t* __restrict__ compact_of,
                             const double2* __restrict__ GO, double* __restrict__ g_model) {
   const int npix = size * size;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= npix) return;
-  const int mx = m / size, my = m - (m / size) * size, cen = size / 2;
+  const int c = blockIdx.y;
+  const int mx = m / size, my = m - (m / size) * size;
   double gr = 0., gi = 0.;
   for (int r = 0; r < R; ++r) {
-    const double ac = rcos[r], as = rsin[r];
-    const double mxc = (double)(mx - cen), myc = (double)(my - cen);
-    const int x0 = (int)rint(mxc * ac + myc * as + cen);
-    const int y0 = (int)rint(-mxc * as + myc * ac + cen);
-    for (int dx = -1; dx <= 1; ++dx)
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int x = x0 + dx, y = y0 + dy;
-        if (x < 0 || x >= size || y < 0 || y >= size) continue;
-        const int j = compact_of[x * size + y];
-        if (j < 0) continue;
-        double tx, ty;
-        rot_source(x, y, cen, ac, as, tx, ty);
-        const int ix = (int)floor(tx), iy = (int)floor(ty);
-        if (ix < 0 || ix > size - 2 || iy < 0 || iy > size - 2) continue;
-        const int ox = mx - ix, oy = my - iy;
-        if (ox < 0 || ox > 1 || oy < 0 || oy > 1) continue;
-        const double fx = tx - ix, fy = ty - iy;
-        const double cx = 1. - fx, cy = 1. - fy;
-        const double w = (ox ? fx : cx) * (oy ? fy : cy);
-        const double2 g = GO[(int64_t)j * Rs + r];
-        gr += g.x * w;
-        gi += g.y * w;
-      }
+    const int h = c * R + r;
+    for_each_source(size, mx, my, rcos[r], rsin[r], compact_of, [&](int j, double w) {
+      const double2 g = GO[(int64_t)j * Hs + h];
+      gr += g.x * w;
+      gi += g.y * w;
+    });
   }
-  g_model[2 * m] = gr;
-  g_model[2 * m + 1] = gi;
+  const int64_t o = (int64_t)c * npix + m;
+  g_model[2 * o] = gr;
+  g_model[2 * o + 1] = gi;
 }
 
 template <typename T>
@@ -692,31 +741,55 @@ int dupload(T** p, const std::vector<T>& v) {
   return kOk;
 }
 
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// Scored frames as CSR in both orders (device pointers).
+struct Csr {
+  const int64_t* fo = nullptr;   // n + 1 frame offsets
+  const int32_t* fj = nullptr;   // compact pixel of each entry
+  const float* fk = nullptr;     // count of each entry
+  const int64_t* po = nullptr;   // K + 1 pixel offsets
+  const int32_t* pd = nullptr;   // frame (local to the CSR) of each entry
+  const float* pk = nullptr;
+  int64_t n = 0;
+};
+
 }  // namespace
 
 struct rsp_rot {
-  int device = 0, size = 0, npix = 0, R = 0, Rs = 0, K = 0, nrad = 0;
+  int device = 0, size = 0, npix = 0, R = 0, C = 1, H = 0, Hs = 0, NT = 0, K = 0, nrad = 0;
   double rel = 1.;
   int64_t Kp = 0, Mpad = 0;
   std::vector<int32_t> scored, compact_of;
   double *d_rcos = nullptr, *d_rsin = nullptr, *d_rad = nullptr;
   int32_t *d_scored = nullptr, *d_jrad = nullptr, *d_compact_of = nullptr;
-  // frames
+  // resident frames
   int64_t N = 0, nnz = 0;
   int64_t *d_fo = nullptr, *d_po = nullptr;
   int32_t *d_fj = nullptr, *d_pd = nullptr;
   float *d_fk = nullptr, *d_pk = nullptr;
-  // work
+  // forward work
   int mode = kSplit, mode_alloc = -1;
   int64_t chunk = 0, chunk_alloc = 0;
   void *d_A = nullptr, *d_OT = nullptr, *d_B = nullptr, *d_X = nullptr;
   double *d_normO = nullptr, *d_part = nullptr, *d_normR = nullptr;
+  // merge
   double *d_VT = nullptr, *d_wsum = nullptr, *d_wpart = nullptr;
+  int64_t wpart_alloc = 0;
   // adjoint (FP64 operands of its own, so that the forward mode's buffers stay)
   int64_t adj_chunk = 0;
   double *d_aA = nullptr, *d_aB = nullptr, *d_aObar = nullptr, *d_aQ = nullptr;
   double *d_aGT = nullptr, *d_apart = nullptr, *d_anormO = nullptr, *d_anormR = nullptr;
   double2 *d_aOT = nullptr, *d_aGO = nullptr;
+  // streamed pass
+  int pass_flags = -1;
+  int64_t pass_max = 0, pc_alloc = 0;
+  bool pass_first = true;
+  double* d_Pc = nullptr;
   // timing
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -726,24 +799,66 @@ struct rsp_rot {
 namespace {
 
 void free_work(rsp_rot* p) {
-  void* ptrs[] = {p->d_A, p->d_OT, p->d_B, p->d_X, p->d_normO, p->d_part, p->d_normR};
-  for (void* q : ptrs)
-    if (q) (void)hipFree(q);
-  p->d_A = p->d_OT = p->d_B = p->d_X = nullptr;
-  p->d_normO = p->d_part = p->d_normR = nullptr;
+  dfree(p->d_A);
+  dfree(p->d_OT);
+  dfree(p->d_B);
+  dfree(p->d_X);
+  dfree(p->d_normO);
+  dfree(p->d_part);
+  dfree(p->d_normR);
   p->mode_alloc = -1;
   p->chunk_alloc = 0;
 }
 
 void free_adj(rsp_rot* p) {
-  void* ptrs[] = {p->d_aA, p->d_aB, p->d_aObar, p->d_aQ, p->d_aGT, p->d_apart,
-                  p->d_anormO, p->d_anormR, p->d_aOT, p->d_aGO};
-  for (void* q : ptrs)
-    if (q) (void)hipFree(q);
-  p->d_aA = p->d_aB = p->d_aObar = p->d_aQ = p->d_aGT = p->d_apart = nullptr;
-  p->d_anormO = p->d_anormR = nullptr;
-  p->d_aOT = p->d_aGO = nullptr;
+  dfree(p->d_aA);
+  dfree(p->d_aB);
+  dfree(p->d_aObar);
+  dfree(p->d_aQ);
+  dfree(p->d_aGT);
+  dfree(p->d_apart);
+  dfree(p->d_anormO);
+  dfree(p->d_anormR);
+  dfree(p->d_aOT);
+  dfree(p->d_aGO);
   p->adj_chunk = 0;
+}
+
+void free_merge(rsp_rot* p) {
+  dfree(p->d_VT);
+  dfree(p->d_wsum);
+  dfree(p->d_wpart);
+  p->wpart_alloc = 0;
+}
+
+int64_t elem_bytes(int mode) { return mode == kFP64 ? 8 : 4; }
+
+int ensure_work(rsp_rot* p, int64_t chunk) {
+  if (p->mode_alloc == p->mode && p->chunk_alloc == chunk) return kOk;
+  free_work(p);
+  const int64_t eb = elem_bytes(p->mode);
+  const int64_t Bpad = xterm_pad_n(chunk);
+  const int64_t ntiles = ceil_div(p->K, 64);
+  const int64_t ot_bytes = (int64_t)p->K * p->Hs * (p->mode == kFP64 ? 16 : 8);
+  int rc;
+  char** A = reinterpret_cast<char**>(&p->d_A);
+  char** B = reinterpret_cast<char**>(&p->d_B);
+  char** OT = reinterpret_cast<char**>(&p->d_OT);
+  char** X = reinterpret_cast<char**>(&p->d_X);
+  if ((rc = dalloc(A, p->Mpad * p->Kp * eb)) || (rc = dalloc(B, Bpad * p->Kp * eb)) ||
+      (rc = dalloc(OT, ot_bytes)) || (rc = dalloc(X, chunk * p->Mpad * eb)) ||
+      (rc = dalloc(&p->d_normO, p->H)) || (rc = dalloc(&p->d_part, ntiles * p->H)) ||
+      (rc = dalloc(&p->d_normR, chunk))) {
+    free_work(p);
+    return rc;
+  }
+  // padding rows / columns of the operands stay zero
+  RSP_HIP(hipMemset(p->d_A, 0, (size_t)(p->Mpad * p->Kp * eb)));
+  RSP_HIP(hipMemset(p->d_B, 0, (size_t)(Bpad * p->Kp * eb)));
+  RSP_HIP(hipMemset(p->d_OT, 0, (size_t)ot_bytes));
+  p->mode_alloc = p->mode;
+  p->chunk_alloc = chunk;
+  return kOk;
 }
 
 int ensure_adj(rsp_rot* p, int64_t chunk) {
@@ -752,185 +867,383 @@ int ensure_adj(rsp_rot* p, int64_t chunk) {
   const int64_t ntiles = ceil_div(p->K, 64);
   int rc;
   if ((rc = dalloc(&p->d_aA, p->Mpad * p->Kp)) || (rc = dalloc(&p->d_aB, chunk * p->Kp)) ||
-      (rc = dalloc(&p->d_aObar, chunk * p->Kp)) || (rc = dalloc(&p->d_aQ, chunk * p->R)) ||
-      (rc = dalloc(&p->d_aGT, (int64_t)p->R * p->Kp)) ||
-      (rc = dalloc(&p->d_apart, ntiles * p->R)) || (rc = dalloc(&p->d_anormO, p->R)) ||
+      (rc = dalloc(&p->d_aObar, chunk * p->Kp)) || (rc = dalloc(&p->d_aQ, chunk * p->H)) ||
+      (rc = dalloc(&p->d_aGT, (int64_t)p->H * p->Kp)) ||
+      (rc = dalloc(&p->d_apart, ntiles * p->H)) || (rc = dalloc(&p->d_anormO, p->H)) ||
       (rc = dalloc(&p->d_anormR, chunk)) ||
-      (rc = dalloc(&p->d_aOT, (int64_t)p->K * p->Rs)) ||
-      (rc = dalloc(&p->d_aGO, (int64_t)p->K * p->Rs))) {
+      (rc = dalloc(&p->d_aOT, (int64_t)p->K * p->Hs)) ||
+      (rc = dalloc(&p->d_aGO, (int64_t)p->K * p->Hs))) {
     free_adj(p);
     return rc;
   }
-  // operand padding (rows >= R of A, columns >= 2K of A and B) stays zero
+  // operand padding (rows >= H of A, columns >= 2K of A and B) stays zero
   RSP_HIP(hipMemset(p->d_aA, 0, sizeof(double) * (size_t)(p->Mpad * p->Kp)));
   RSP_HIP(hipMemset(p->d_aB, 0, sizeof(double) * (size_t)(chunk * p->Kp)));
-  RSP_HIP(hipMemset(p->d_aOT, 0, sizeof(double2) * (size_t)((int64_t)p->K * p->Rs)));
+  RSP_HIP(hipMemset(p->d_aOT, 0, sizeof(double2) * (size_t)((int64_t)p->K * p->Hs)));
   p->adj_chunk = chunk;
   return kOk;
 }
 
-template <int NT>
-void launch_adj_frame_nt(rsp_rot* p, size_t shmem, const double* prob, const double* sx,
-                         const double* sy, const double* dia, const double* scales, int64_t d0,
-                         int64_t n, double* gsx, double* gsy, double* gdia, hipStream_t st) {
-  hipLaunchKernelGGL((k_adj_frame<NT>), dim3((unsigned)n), dim3(256), shmem, st, p->size, p->K,
-                     p->R, p->Rs, p->rel, p->d_scored, p->d_jrad, p->d_rad, p->nrad, p->d_fo,
-                     p->d_fj, p->d_fk, p->d_aOT, prob, p->d_aObar, p->Kp, sx, sy, dia, scales,
-                     d0, gsx, gsy, gdia);
-}
-
-int launch_adj_frame(rsp_rot* p, size_t shmem, const double* prob, const double* sx,
-                     const double* sy, const double* dia, const double* scales, int64_t d0,
-                     int64_t n, double* gsx, double* gsy, double* gdia, hipStream_t st) {
-  switch (p->Rs / 64) {
-#define RSP_NT(k)                                                                        \
-  case k:                                                                                \
-    launch_adj_frame_nt<k>(p, shmem, prob, sx, sy, dia, scales, d0, n, gsx, gsy, gdia, st); \
-    break;
-    RSP_NT(1) RSP_NT(2) RSP_NT(3) RSP_NT(4) RSP_NT(5) RSP_NT(6) RSP_NT(7) RSP_NT(8)
-    RSP_NT(9) RSP_NT(10) RSP_NT(11) RSP_NT(12)
-#undef RSP_NT
-    default: return fail(kErrUnsupported, "rotation form supports at most 768 rotations");
-  }
-  RSP_LAUNCH_CHECK("k_adj_frame");
-  return kOk;
-}
-
-int launch_adj_pixel(rsp_rot* p, const double* prob, const double* sx, const double* sy,
-                     const double* dia, const double* scales, hipStream_t st) {
-  const dim3 g((unsigned)p->K), b(256);
-  switch (ceil_div(p->Rs, 256)) {
-    case 1:
-      hipLaunchKernelGGL(k_adj_pixel<1>, g, b, 0, st, p->size, p->R, p->Rs, p->rel, p->d_scored,
-                         p->d_po, p->d_pd, p->d_pk, p->d_aOT, prob, p->d_aGT, p->d_wsum, sx, sy,
-                         dia, scales, p->d_aGO);
-      break;
-    case 2:
-      hipLaunchKernelGGL(k_adj_pixel<2>, g, b, 0, st, p->size, p->R, p->Rs, p->rel, p->d_scored,
-                         p->d_po, p->d_pd, p->d_pk, p->d_aOT, prob, p->d_aGT, p->d_wsum, sx, sy,
-                         dia, scales, p->d_aGO);
-      break;
-    case 3:
-      hipLaunchKernelGGL(k_adj_pixel<3>, g, b, 0, st, p->size, p->R, p->Rs, p->rel, p->d_scored,
-                         p->d_po, p->d_pd, p->d_pk, p->d_aOT, prob, p->d_aGT, p->d_wsum, sx, sy,
-                         dia, scales, p->d_aGO);
-      break;
-    default: return fail(kErrUnsupported, "rotation form supports at most 768 rotations");
-  }
-  RSP_LAUNCH_CHECK("k_adj_pixel");
-  return kOk;
-}
-
-// ws_r = sum_d P[d, r] scale_d into p->d_wsum (fixed summation order).
-int launch_wsum(rsp_rot* p, const double* prob, const double* scales, hipStream_t st) {
-  const int64_t per = 1024;
-  const int64_t nb = ceil_div(p->N, per);
+int ensure_merge(rsp_rot* p, int64_t nblocks) {
   int rc;
-  if (!p->d_wsum && (rc = dalloc(&p->d_wsum, p->R))) return rc;
-  if ((rc = dalloc(&p->d_wpart, nb * p->R))) return rc;
-  hipLaunchKernelGGL(k_wsum_part, dim3((unsigned)nb, (unsigned)ceil_div(p->R, 256)), dim3(256),
-                     0, st, prob, scales, p->N, p->R, per, p->d_wpart);
-  RSP_LAUNCH_CHECK("k_wsum_part");
-  hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)ceil_div(p->R, 256)), dim3(256), 0, st,
-                     p->d_wpart, (int)nb, p->R, p->d_wsum);
-  RSP_LAUNCH_CHECK("k_norm_reduce");
-  return kOk;
-}
-
-int64_t elem_bytes(int mode) { return mode == kFP64 ? 8 : (mode == kSplit ? 4 : 4); }
-
-int ensure_work(rsp_rot* p, int64_t chunk) {
-  if (p->mode_alloc == p->mode && p->chunk_alloc == chunk) return kOk;
-  free_work(p);
-  const int64_t eb = elem_bytes(p->mode);
-  const int64_t Bpad = xterm_pad_n(chunk);
-  const int64_t ntiles = ceil_div(p->K, 64);
-  int rc;
-  char** A = reinterpret_cast<char**>(&p->d_A);
-  char** B = reinterpret_cast<char**>(&p->d_B);
-  char** OT = reinterpret_cast<char**>(&p->d_OT);
-  char** X = reinterpret_cast<char**>(&p->d_X);
-  if ((rc = dalloc(A, p->Mpad * p->Kp * eb)) || (rc = dalloc(B, Bpad * p->Kp * eb)) ||
-      (rc = dalloc(OT, (int64_t)p->K * p->Rs * (p->mode == kFP64 ? 16 : 8))) ||
-      (rc = dalloc(X, chunk * p->Mpad * (p->mode == kFP64 ? 8 : 4))) ||
-      (rc = dalloc(&p->d_normO, p->R)) || (rc = dalloc(&p->d_part, ntiles * p->R)) ||
-      (rc = dalloc(&p->d_normR, chunk))) {
-    free_work(p);
-    return rc;
+  if (!p->d_VT && (rc = dalloc(&p->d_VT, (int64_t)p->K * p->Hs))) return rc;
+  if (!p->d_wsum && (rc = dalloc(&p->d_wsum, p->H))) return rc;
+  if (p->wpart_alloc < nblocks) {
+    if ((rc = dalloc(&p->d_wpart, nblocks * p->H))) return rc;
+    p->wpart_alloc = nblocks;
   }
-  // padding rows / columns of the operands stay zero
-  RSP_HIP(hipMemset(p->d_A, 0, (size_t)(p->Mpad * p->Kp * eb)));
-  RSP_HIP(hipMemset(p->d_B, 0, (size_t)(Bpad * p->Kp * eb)));
-  RSP_HIP(hipMemset(p->d_OT, 0, (size_t)((int64_t)p->K * p->Rs * (p->mode == kFP64 ? 16 : 8))));
-  p->mode_alloc = p->mode;
-  p->chunk_alloc = chunk;
   return kOk;
 }
 
-// O_r rows (A), the pixel-major copy (OT) and |O_r|^2 (normO) into the given buffers.
+// O_h rows (A), the pixel-major copy (OT) and |O_h|^2 (normO) into the given buffers.
 template <int MODE>
 int launch_slices_to(rsp_rot* p, const double* model_c, void* A, void* OT, double* part,
                      double* normO, hipStream_t st) {
-  dim3 g((unsigned)ceil_div(p->K, 64), (unsigned)ceil_div(p->R, 32));
+  dim3 g((unsigned)ceil_div(p->K, 64), (unsigned)ceil_div(p->H, 32));
   hipLaunchKernelGGL(k_rot_slices<MODE>, g, dim3(256), 0, st, model_c, p->size, p->K, p->R,
-                     p->d_scored, p->d_rcos, p->d_rsin, A, p->Mpad, p->Kp, OT, p->Rs, part);
+                     p->H, p->d_scored, p->d_rcos, p->d_rsin, A, p->Mpad, p->Kp, OT, p->Hs,
+                     part);
   RSP_LAUNCH_CHECK("k_rot_slices");
-  hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)ceil_div(p->R, 256)), dim3(256), 0, st,
-                     part, (int)ceil_div(p->K, 64), p->R, normO);
+  hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)ceil_div(p->H, 256)), dim3(256), 0, st,
+                     part, (int)ceil_div(p->K, 64), p->H, normO, 0);
   RSP_LAUNCH_CHECK("k_norm_reduce");
   return kOk;
 }
 
-template <int MODE>
 int launch_slices(rsp_rot* p, const double* model_c, hipStream_t st) {
-  return launch_slices_to<MODE>(p, model_c, p->d_A, p->d_OT, p->d_part, p->d_normO, st);
+  if (p->mode == kFP32)
+    return launch_slices_to<kFP32>(p, model_c, p->d_A, p->d_OT, p->d_part, p->d_normO, st);
+  if (p->mode == kSplit)
+    return launch_slices_to<kSplit>(p, model_c, p->d_A, p->d_OT, p->d_part, p->d_normO, st);
+  return launch_slices_to<kFP64>(p, model_c, p->d_A, p->d_OT, p->d_part, p->d_normO, st);
 }
 
-// R_d rows of frames [d0, d0 + n) into B (Bpad rows per bf16 plane) and normR.
+// R_d rows of frames [d0, d0 + n) of arrays of N frames into B (Bpad rows per bf16 plane).
 template <int MODE>
 int launch_ref_to(rsp_rot* p, const double* sx, const double* sy, const double* dia, int64_t d0,
-                  int64_t n, void* B, int64_t Bpad, double* normR, hipStream_t st) {
+                  int64_t n, int64_t N, void* B, int64_t Bpad, double* normR, hipStream_t st) {
   const size_t shmem =
       sizeof(double) * (size_t)((p->nrad + 1) & ~1) + 2 * sizeof(double2) * (size_t)p->size;
   hipLaunchKernelGGL(k_ref_rows<MODE>, dim3((unsigned)n), dim3(256), shmem, st, p->size, p->K,
-                     p->d_scored, p->d_jrad, p->d_rad, p->nrad, p->rel, sx, sy, dia, d0, p->N, B,
+                     p->d_scored, p->d_jrad, p->d_rad, p->nrad, p->rel, sx, sy, dia, d0, N, B,
                      Bpad, p->Kp, normR);
   RSP_LAUNCH_CHECK("k_ref_rows");
   return kOk;
 }
 
-template <int MODE>
-int launch_ref(rsp_rot* p, const double* sx, const double* sy, const double* dia, int64_t d0,
-               int64_t n, hipStream_t st) {
-  return launch_ref_to<MODE>(p, sx, sy, dia, d0, n, p->d_B, xterm_pad_n(p->chunk_alloc),
-                             p->d_normR, st);
-}
-
 template <typename T, int NT>
-void launch_ll_nt(rsp_rot* p, const double* sx, const double* sy, const double* dia,
-                  const double* scales, int64_t d0, int64_t d1, double* ll, double* prob,
-                  double* logz, hipStream_t st) {
+void launch_ll_nt(rsp_rot* p, const Csr& f, const double* sx, const double* sy,
+                  const double* dia, const double* scales, int64_t d0, int64_t d1, double* ll,
+                  double* prob, double* logz, int32_t* best, hipStream_t st) {
   hipLaunchKernelGGL((k_rot_ll<T, NT>), dim3((unsigned)ceil_div(d1 - d0, 4)), dim3(256), 0, st,
-                     p->size, p->R, p->Rs, p->rel, p->d_fo, p->d_fj, p->d_fk, p->d_scored,
-                     p->d_OT, p->d_normO, p->d_normR, p->d_X, p->Mpad, sx, sy, dia, scales, d0,
-                     d1, ll, prob, logz);
+                     p->size, p->H, p->Hs, p->rel, f.fo, f.fj, f.fk, p->d_scored, p->d_OT,
+                     p->d_normO, p->d_normR, p->d_X, p->Mpad, sx, sy, dia, scales, d0, d1, ll,
+                     prob, logz, best);
 }
 
 template <typename T>
-int launch_ll(rsp_rot* p, const double* sx, const double* sy, const double* dia,
+int launch_ll(rsp_rot* p, const Csr& f, const double* sx, const double* sy, const double* dia,
               const double* scales, int64_t d0, int64_t d1, double* ll, double* prob,
-              double* logz, hipStream_t st) {
-  const int nt = p->Rs / 64;
-  switch (nt) {
-#define RSP_NT(n) \
-  case n: launch_ll_nt<T, n>(p, sx, sy, dia, scales, d0, d1, ll, prob, logz, st); break;
-    RSP_NT(1) RSP_NT(2) RSP_NT(3) RSP_NT(4) RSP_NT(5) RSP_NT(6) RSP_NT(7) RSP_NT(8)
-    RSP_NT(9) RSP_NT(10) RSP_NT(11) RSP_NT(12)
+              double* logz, int32_t* best, hipStream_t st) {
+  switch (p->NT) {
+#define RSP_NT(n)                                                                          \
+  case n:                                                                                  \
+    launch_ll_nt<T, n>(p, f, sx, sy, dia, scales, d0, d1, ll, prob, logz, best, st);      \
+    break;
+    RSP_NT(1) RSP_NT(2) RSP_NT(3) RSP_NT(4) RSP_NT(5) RSP_NT(6) RSP_NT(8) RSP_NT(10)
+    RSP_NT(12) RSP_NT(16) RSP_NT(20) RSP_NT(24)
 #undef RSP_NT
-    default: return fail(kErrUnsupported, "rotation form supports at most 768 rotations");
+    default: return fail(kErrUnsupported, "hypothesis count not supported");
   }
   RSP_LAUNCH_CHECK("k_rot_ll");
   return kOk;
+}
+
+int ensure_events(rsp_rot* p) {
+  for (auto& e : p->ev)
+    if (!e) RSP_HIP(hipEventCreate(&e));
+  return kOk;
+}
+
+// E-step over the frames of `f` (per-frame arrays indexed like f's frames), in GEMM chunks of
+// p->chunk_alloc frames.  The slices of the forward mode must be current.
+int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, const double* dia,
+              const double* scales, double* ll, double* prob, double* logz, int32_t* best,
+              hipStream_t st) {
+  const int64_t chunk = p->chunk_alloc;
+  const int64_t Bpad = xterm_pad_n(chunk);
+  int rc = kOk;
+  if (p->timing && (rc = ensure_events(p))) return rc;
+  for (int64_t d0 = 0; d0 < f.n; d0 += chunk) {
+    const int64_t n = std::min(chunk, f.n - d0);
+    if (p->timing) RSP_HIP(hipEventRecord(p->ev[2], st));
+    if (p->mode == kFP32)
+      rc = launch_ref_to<kFP32>(p, sx, sy, dia, d0, n, f.n, p->d_B, Bpad, p->d_normR, st);
+    else if (p->mode == kSplit)
+      rc = launch_ref_to<kSplit>(p, sx, sy, dia, d0, n, f.n, p->d_B, Bpad, p->d_normR, st);
+    else
+      rc = launch_ref_to<kFP64>(p, sx, sy, dia, d0, n, f.n, p->d_B, Bpad, p->d_normR, st);
+    if (rc) return rc;
+    if (p->timing) RSP_HIP(hipEventRecord(p->ev[3], st));
+    if (p->mode == kFP32) {
+      rc = xterm_gemm_blas32((const float*)p->d_A, (const float*)p->d_B, (float*)p->d_X, p->Mpad,
+                             n, p->Kp, p->Kp, p->Kp, p->Mpad, st);
+    } else if (p->mode == kFP64) {
+      rc = xterm_gemm_blas64((const double*)p->d_A, (const double*)p->d_B, (double*)p->d_X,
+                             p->Mpad, n, p->Kp, p->Kp, p->Kp, p->Mpad, st);
+    } else {
+      const __bf16* ah = (const __bf16*)p->d_A;
+      const __bf16* bh = (const __bf16*)p->d_B;
+      rc = xterm_gemm_split(ah, ah + p->Mpad * p->Kp, bh, bh + Bpad * p->Kp, (float*)p->d_X,
+                            p->Mpad, n, p->Kp, p->Mpad, st);
+    }
+    if (rc) return rc;
+    if (p->timing) RSP_HIP(hipEventRecord(p->ev[4], st));
+    if (p->mode == kFP64)
+      rc = launch_ll<double>(p, f, sx, sy, dia, scales, d0, d0 + n, ll, prob, logz, best, st);
+    else
+      rc = launch_ll<float>(p, f, sx, sy, dia, scales, d0, d0 + n, ll, prob, logz, best, st);
+    if (rc) return rc;
+    if (p->timing) {
+      RSP_HIP(hipEventRecord(p->ev[5], st));
+      RSP_HIP(hipEventSynchronize(p->ev[5]));
+      float a = 0.f, b = 0.f, c = 0.f;
+      RSP_HIP(hipEventElapsedTime(&a, p->ev[2], p->ev[3]));
+      RSP_HIP(hipEventElapsedTime(&b, p->ev[3], p->ev[4]));
+      RSP_HIP(hipEventElapsedTime(&c, p->ev[4], p->ev[5]));
+      p->t_ms[1] += a;
+      p->t_ms[2] += b;
+      p->t_ms[3] += c;
+      p->t_ms[4] += 1.;
+    }
+  }
+  return kOk;
+}
+
+// ws_h (+)= sum_d P[d, h] scale_d over n frames into p->d_wsum (fixed summation order).
+int run_wsum(rsp_rot* p, const double* prob, const double* scales, int64_t n, bool accumulate,
+             hipStream_t st) {
+  const int64_t per = 1024;
+  const int64_t nb = ceil_div(n, per);
+  int rc = ensure_merge(p, nb);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_wsum_part, dim3((unsigned)nb, (unsigned)ceil_div(p->H, 256)), dim3(256),
+                     0, st, prob, scales, n, p->H, per, p->d_wpart);
+  RSP_LAUNCH_CHECK("k_wsum_part");
+  hipLaunchKernelGGL(k_norm_reduce, dim3((unsigned)ceil_div(p->H, 256)), dim3(256), 0, st,
+                     p->d_wpart, (int)nb, p->H, p->d_wsum, accumulate ? 1 : 0);
+  RSP_LAUNCH_CHECK("k_norm_reduce");
+  return kOk;
+}
+
+int run_merge_view(rsp_rot* p, const Csr& f, const double* prob, bool accumulate,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_merge_view, dim3((unsigned)p->K), dim3(256), 0, st, f.po, f.pd, f.pk,
+                     prob, p->H, p->Hs, p->d_VT, accumulate ? 1 : 0);
+  RSP_LAUNCH_CHECK("k_merge_view");
+  return kOk;
+}
+
+int run_merge_model(rsp_rot* p, double* model_out, double* acc_out, double* wts_out,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_merge_model, dim3((unsigned)ceil_div(p->npix, 256), (unsigned)p->C),
+                     dim3(256), 0, st, p->size, p->R, p->Hs, p->d_rcos, p->d_rsin,
+                     p->d_compact_of, p->d_VT, p->d_wsum, model_out, acc_out, wts_out);
+  RSP_LAUNCH_CHECK("k_merge_model");
+  return kOk;
+}
+
+size_t adj_frame_shmem(const rsp_rot* p) {
+  return sizeof(double) * (size_t)((2 * p->nrad + 1) & ~1) +
+         2 * sizeof(double2) * (size_t)p->size;
+}
+
+int check_adj_lds(const rsp_rot* p) {
+  int lds_max = 0;
+  RSP_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
+  if (adj_frame_shmem(p) + 64 > (size_t)lds_max)
+    return fail(kErrSize, "adjoint: reference tables exceed the LDS of a work-group");
+  return kOk;
+}
+
+template <int NT>
+void launch_adj_frame_nt(rsp_rot* p, const Csr& f, const double* prob, const double* sx,
+                         const double* sy, const double* dia, const double* scales, int64_t d0,
+                         int64_t n, double* gsx, double* gsy, double* gdia, hipStream_t st) {
+  hipLaunchKernelGGL((k_adj_frame<NT>), dim3((unsigned)n), dim3(256), adj_frame_shmem(p), st,
+                     p->size, p->K, p->H, p->Hs, p->rel, p->d_scored, p->d_jrad, p->d_rad,
+                     p->nrad, f.fo, f.fj, f.fk, p->d_aOT, prob, p->d_aObar, p->Kp, sx, sy, dia,
+                     scales, d0, gsx, gsy, gdia);
+}
+
+int launch_adj_frame(rsp_rot* p, const Csr& f, const double* prob, const double* sx,
+                     const double* sy, const double* dia, const double* scales, int64_t d0,
+                     int64_t n, double* gsx, double* gsy, double* gdia, hipStream_t st) {
+  switch (p->NT) {
+#define RSP_NT(k)                                                                          \
+  case k:                                                                                  \
+    launch_adj_frame_nt<k>(p, f, prob, sx, sy, dia, scales, d0, n, gsx, gsy, gdia, st);   \
+    break;
+    RSP_NT(1) RSP_NT(2) RSP_NT(3) RSP_NT(4) RSP_NT(5) RSP_NT(6) RSP_NT(8) RSP_NT(10)
+    RSP_NT(12) RSP_NT(16) RSP_NT(20) RSP_NT(24)
+#undef RSP_NT
+    default: return fail(kErrUnsupported, "hypothesis count not supported");
+  }
+  RSP_LAUNCH_CHECK("k_adj_frame");
+  return kOk;
+}
+
+// Adjoint sub-chunks over the frames of `f`: the dense model term GT (+)= Q^T B and/or the
+// frame gradients.  The FP64 slices (d_aA, d_aOT) must be current.
+int run_adj_frames(rsp_rot* p, const Csr& f, const double* prob, const double* sx,
+                   const double* sy, const double* dia, const double* scales, double* gsx,
+                   double* gsy, double* gdia, bool want_frame, bool want_model, bool gt_first,
+                   hipStream_t st) {
+  const int64_t chunk = p->adj_chunk;
+  int rc;
+  for (int64_t d0 = 0; d0 < f.n; d0 += chunk) {
+    const int64_t n = std::min(chunk, f.n - d0);
+    if ((rc = launch_ref_to<kFP64>(p, sx, sy, dia, d0, n, f.n, p->d_aB, chunk, p->d_anormR, st)))
+      return rc;
+    if (want_model) {
+      hipLaunchKernelGGL(k_scaled_prob, dim3((unsigned)ceil_div(n * p->H, 256)), dim3(256), 0,
+                         st, prob, scales, d0, n, p->H, p->d_aQ);
+      RSP_LAUNCH_CHECK("k_scaled_prob");
+      // GT (H x Kp, column-major) (+)= Q^T B : GT[k * H + h] = sum_d Q[d, h] B[d, k]
+      if ((rc = xterm_dgemm(false, true, p->H, p->Kp, n, 1., p->d_aQ, p->H, p->d_aB, p->Kp,
+                            (gt_first && d0 == 0) ? 0. : 1., p->d_aGT, p->H, st)))
+        return rc;
+    }
+    if (want_frame) {
+      // Obar (n x Kp, row-major) = P A : Obar[d * Kp + k] = sum_h P[d, h] A[h, k]
+      if ((rc = xterm_dgemm(false, false, p->Kp, n, p->H, 1., p->d_aA, p->Kp, prob + d0 * p->H,
+                            p->H, 0., p->d_aObar, p->Kp, st)))
+        return rc;
+      if ((rc = launch_adj_frame(p, f, prob, sx, sy, dia, scales, d0, n, gsx, gsy, gdia, st)))
+        return rc;
+    }
+  }
+  return kOk;
+}
+
+int run_adj_pixel(rsp_rot* p, const Csr& f, const double* prob, const double* sx,
+                  const double* sy, const double* dia, const double* scales, bool accumulate,
+                  hipStream_t st) {
+  const dim3 g((unsigned)p->K), b(256);
+  const int acc = accumulate ? 1 : 0;
+  switch (ceil_div(p->Hs, 256)) {
+#define RSP_NU(u)                                                                          \
+  case u:                                                                                  \
+    hipLaunchKernelGGL(k_adj_pixel_acc<u>, g, b, 0, st, p->size, p->H, p->Hs, p->rel,       \
+                       p->d_scored, f.po, f.pd, f.pk, p->d_aOT, prob, sx, sy, dia, scales,  \
+                       p->d_aGO, acc);                                                      \
+    break;
+    RSP_NU(1) RSP_NU(2) RSP_NU(3) RSP_NU(4) RSP_NU(5) RSP_NU(6)
+#undef RSP_NU
+    default: return fail(kErrUnsupported, "hypothesis count not supported");
+  }
+  RSP_LAUNCH_CHECK("k_adj_pixel_acc");
+  return kOk;
+}
+
+int run_adj_finish(rsp_rot* p, double* g_model, hipStream_t st) {
+  const int64_t n = (int64_t)p->K * p->Hs;
+  hipLaunchKernelGGL(k_adj_combine, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, p->K,
+                     p->H, p->Hs, p->d_aOT, p->d_aGT, p->d_wsum, p->d_aGO);
+  RSP_LAUNCH_CHECK("k_adj_combine");
+  hipLaunchKernelGGL(k_adj_model, dim3((unsigned)ceil_div(p->npix, 256), (unsigned)p->C),
+                     dim3(256), 0, st, p->size, p->R, p->Hs, p->d_rcos, p->d_rsin,
+                     p->d_compact_of, p->d_aGO, g_model);
+  RSP_LAUNCH_CHECK("k_adj_model");
+  return kOk;
+}
+
+int64_t default_chunk(const rsp_rot* p, int64_t nframes) {
+  int64_t chunk = p->chunk;
+  if (chunk <= 0) {
+    // operand rows of a chunk within about 8 GB
+    chunk = std::max<int64_t>(128, (int64_t)(8e9 / (double)(p->Kp * elem_bytes(p->mode))) /
+                                       128 * 128);
+  }
+  return std::min(chunk, xterm_pad_n(nframes));
+}
+
+Csr resident(const rsp_rot* p) {
+  Csr f;
+  f.fo = p->d_fo;
+  f.fj = p->d_fj;
+  f.fk = p->d_fk;
+  f.po = p->d_po;
+  f.pd = p->d_pd;
+  f.pk = p->d_pk;
+  f.n = p->N;
+  return f;
+}
+
+// Host packing of frames (dset layout) into the scored CSR of both orders.
+int pack_frames(const rsp_rot* p, int64_t N, const int32_t* ones, const int32_t* multi,
+                const int32_t* place_ones, const int32_t* place_multi,
+                const int32_t* count_multi, std::vector<int64_t>& fo, std::vector<int32_t>& fj,
+                std::vector<float>& fk, std::vector<int64_t>& po, std::vector<int32_t>& pd,
+                std::vector<float>& pk) {
+  fo.assign(N + 1, 0);
+  int64_t no = 0, nm = 0;
+  for (int64_t d = 0; d < N; ++d) {
+    if (ones[d] < 0 || multi[d] < 0) return fail(kErrSize, "negative photon count");
+    no += ones[d];
+    nm += multi[d];
+  }
+  if ((no && !place_ones) || (nm && (!place_multi || !count_multi)))
+    return fail(kErrNull, "null photon list");
+  fj.clear();
+  fk.clear();
+  fj.reserve((size_t)(no + nm));
+  fk.reserve((size_t)(no + nm));
+  int64_t io = 0, im = 0;
+  for (int64_t d = 0; d < N; ++d) {
+    for (int q = 0; q < ones[d]; ++q, ++io) {
+      const int t = place_ones[io];
+      if (t < 0 || t >= p->npix) return fail(kErrSize, "photon index out of range");
+      const int j = p->compact_of[t];
+      if (j >= 0) {
+        fj.push_back(j);
+        fk.push_back(1.f);
+      }
+    }
+    for (int q = 0; q < multi[d]; ++q, ++im) {
+      const int t = place_multi[im];
+      if (t < 0 || t >= p->npix) return fail(kErrSize, "photon index out of range");
+      const int j = p->compact_of[t];
+      if (j >= 0) {
+        fj.push_back(j);
+        fk.push_back((float)count_multi[im]);
+      }
+    }
+    fo[d + 1] = (int64_t)fj.size();
+  }
+  const int64_t nnz = (int64_t)fj.size();
+  // pixel-major lists (stable in frame order)
+  po.assign(p->K + 1, 0);
+  for (int64_t e = 0; e < nnz; ++e) ++po[fj[e] + 1];
+  for (int j = 0; j < p->K; ++j) po[j + 1] += po[j];
+  std::vector<int64_t> fill(po.begin(), po.end() - 1);
+  pd.assign(nnz, 0);
+  pk.assign(nnz, 0.f);
+  for (int64_t d = 0; d < N; ++d)
+    for (int64_t e = fo[d]; e < fo[d + 1]; ++e) {
+      const int64_t q = fill[fj[e]]++;
+      pd[q] = (int32_t)d;
+      pk[q] = fk[e];
+    }
+  return kOk;
+}
+
+void set_hypotheses(rsp_rot* p, int C) {
+  p->C = C;
+  p->H = C * p->R;
+  p->NT = nt_for(p->H);
+  p->Hs = 64 * p->NT;
+  p->Mpad = xterm_pad_m(p->H);
 }
 
 }  // namespace
@@ -942,14 +1255,15 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
   if (!out || !rot_cos || !rot_sin) return fail(kErrNull, "rsp_rot_create: null argument");
   *out = nullptr;
   if (size < 9 || size > 4096) return fail(kErrSize, "rsp_rot_create: size out of range");
-  if (num_rot < 1 || num_rot > 768) return fail(kErrSize, "rsp_rot_create: 1 <= num_rot <= 768");
+  if (num_rot < 1 || num_rot > kMaxH)
+    return fail(kErrSize, "rsp_rot_create: 1 <= num_rot <= 1536");
   RSP_HIP(hipSetDevice(device));
   rsp_rot* p = new rsp_rot();
   p->device = device;
   p->size = (int)size;
   p->npix = (int)(size * size);
   p->R = (int)num_rot;
-  p->Rs = (int)ceil_div(num_rot, 64) * 64;
+  set_hypotheses(p, 1);
   p->rel = rel_scale;
   // scored pixels: probmask == 0 of emc.py:89-98
   const int c = (int)size / 2;
@@ -982,7 +1296,6 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
   for (int j = 0; j < p->K; ++j) jrad[j] = radix[r2_of[j]];
   p->nrad = (int)rad.size();
   p->Kp = xterm_pad_k(2 * (int64_t)p->K);
-  p->Mpad = xterm_pad_m(p->R);
   std::vector<double> rc(rot_cos, rot_cos + num_rot), rs(rot_sin, rot_sin + num_rot);
   int rc_ = kOk;
   if ((rc_ = dupload(&p->d_rcos, rc)) || (rc_ = dupload(&p->d_rsin, rs)) ||
@@ -1004,63 +1317,52 @@ int rsp_rot_set_frames(rsp_rot* p, int64_t num_data, const int32_t* ones, const 
                        const int32_t* count_multi) {
   if (!p || !ones || !multi) return fail(kErrNull, "rsp_rot_set_frames: null argument");
   if (num_data < 1) return fail(kErrSize, "rsp_rot_set_frames: no frames");
+  if (num_data > 0x7fffffff) return fail(kErrSize, "rsp_rot_set_frames: too many frames");
   RSP_HIP(hipSetDevice(p->device));
-  const int64_t N = num_data;
-  std::vector<int64_t> fo(N + 1, 0);
-  int64_t no = 0, nm = 0;
-  for (int64_t d = 0; d < N; ++d) {
-    if (ones[d] < 0 || multi[d] < 0) return fail(kErrSize, "negative photon count");
-    no += ones[d];
-    nm += multi[d];
-  }
-  if ((no && !place_ones) || (nm && (!place_multi || !count_multi)))
-    return fail(kErrNull, "rsp_rot_set_frames: null photon list");
-  std::vector<int32_t> fj;
-  std::vector<float> fk;
-  fj.reserve((size_t)(no + nm));
-  fk.reserve((size_t)(no + nm));
-  int64_t io = 0, im = 0;
-  for (int64_t d = 0; d < N; ++d) {
-    for (int q = 0; q < ones[d]; ++q, ++io) {
-      const int t = place_ones[io];
-      if (t < 0 || t >= p->npix) return fail(kErrSize, "photon index out of range");
-      const int j = p->compact_of[t];
-      if (j >= 0) { fj.push_back(j); fk.push_back(1.f); }
-    }
-    for (int q = 0; q < multi[d]; ++q, ++im) {
-      const int t = place_multi[im];
-      if (t < 0 || t >= p->npix) return fail(kErrSize, "photon index out of range");
-      const int j = p->compact_of[t];
-      if (j >= 0) { fj.push_back(j); fk.push_back((float)count_multi[im]); }
-    }
-    fo[d + 1] = (int64_t)fj.size();
-  }
-  const int64_t nnz = (int64_t)fj.size();
-  // pixel-major lists (stable in frame order)
-  std::vector<int64_t> po(p->K + 1, 0);
-  for (int64_t e = 0; e < nnz; ++e) ++po[fj[e] + 1];
-  for (int j = 0; j < p->K; ++j) po[j + 1] += po[j];
-  std::vector<int64_t> fill(po.begin(), po.end() - 1);
-  std::vector<int32_t> pd(nnz);
-  std::vector<float> pk(nnz);
-  for (int64_t d = 0; d < N; ++d)
-    for (int64_t e = fo[d]; e < fo[d + 1]; ++e) {
-      const int64_t q = fill[fj[e]]++;
-      pd[q] = (int32_t)d;
-      pk[q] = fk[e];
-    }
-  int rc;
+  std::vector<int64_t> fo, po;
+  std::vector<int32_t> fj, pd;
+  std::vector<float> fk, pk;
+  int rc = pack_frames(p, num_data, ones, multi, place_ones, place_multi, count_multi, fo, fj,
+                       fk, po, pd, pk);
+  if (rc) return rc;
   if ((rc = dupload(&p->d_fo, fo)) || (rc = dupload(&p->d_fj, fj)) ||
       (rc = dupload(&p->d_fk, fk)) || (rc = dupload(&p->d_po, po)) ||
       (rc = dupload(&p->d_pd, pd)) || (rc = dupload(&p->d_pk, pk)))
     return rc;
-  p->N = N;
-  p->nnz = nnz;
+  p->N = num_data;
+  p->nnz = (int64_t)fj.size();
+  return kOk;
+}
+
+int rsp_rot_pack(const rsp_rot* p, int64_t num_data, const int32_t* ones, const int32_t* multi,
+                 const int32_t* place_ones, const int32_t* place_multi,
+                 const int32_t* count_multi, int64_t* fo, int32_t* fj, float* fk, int64_t* po,
+                 int32_t* pd, float* pk, int64_t* nnz_out) {
+  if (!p || !ones || !multi || !nnz_out) return fail(kErrNull, "rsp_rot_pack: null argument");
+  if (num_data < 1 || num_data > 0x7fffffff) return fail(kErrSize, "rsp_rot_pack: frame count");
+  std::vector<int64_t> vfo, vpo;
+  std::vector<int32_t> vfj, vpd;
+  std::vector<float> vfk, vpk;
+  int rc = pack_frames(p, num_data, ones, multi, place_ones, place_multi, count_multi, vfo, vfj,
+                       vfk, vpo, vpd, vpk);
+  if (rc) return rc;
+  *nnz_out = (int64_t)vfj.size();
+  if (!fo && !fj && !fk && !po && !pd && !pk) return kOk;   // size query
+  if (!fo || !fj || !fk || !po || !pd || !pk) return fail(kErrNull, "rsp_rot_pack: null output");
+  std::memcpy(fo, vfo.data(), sizeof(int64_t) * vfo.size());
+  std::memcpy(po, vpo.data(), sizeof(int64_t) * vpo.size());
+  if (!vfj.empty()) {
+    std::memcpy(fj, vfj.data(), sizeof(int32_t) * vfj.size());
+    std::memcpy(fk, vfk.data(), sizeof(float) * vfk.size());
+    std::memcpy(pd, vpd.data(), sizeof(int32_t) * vpd.size());
+    std::memcpy(pk, vpk.data(), sizeof(float) * vpk.size());
+  }
   return kOk;
 }
 
 int rsp_rot_set_option(rsp_rot* p, int key, int64_t value) {
   if (!p) return fail(kErrNull, "rsp_rot_set_option: null plan");
+  if (p->pass_flags >= 0) return fail(kErrState, "rsp_rot_set_option: a pass is open");
   switch (key) {
     case 1:
       if (value < 0 || value > 2) return fail(kErrSize, "mode must be 0, 1 or 2");
@@ -1070,6 +1372,17 @@ int rsp_rot_set_option(rsp_rot* p, int key, int64_t value) {
       if (value < 0) return fail(kErrSize, "chunk must be >= 0");
       p->chunk = value;
       return kOk;
+    case 3: {
+      if (value < 1 || value * p->R > kMaxH)
+        return fail(kErrSize, "classes must be >= 1 with classes * num_rot <= 1536");
+      if (value == p->C) return kOk;
+      RSP_HIP(hipSetDevice(p->device));
+      free_work(p);
+      free_adj(p);
+      free_merge(p);
+      set_hypotheses(p, (int)value);
+      return kOk;
+    }
     case 5:
       p->timing = value != 0;
       return kOk;
@@ -1081,30 +1394,26 @@ int rsp_rot_set_option(rsp_rot* p, int key, int64_t value) {
 int rsp_rot_estep(rsp_rot* p, const double* model_c, const double* sx, const double* sy,
                   const double* dia, const double* scales, double* ll, double* prob,
                   double* logz, rsp_stream_t stream_) {
+  return rsp_rot_estep_best(p, model_c, sx, sy, dia, scales, ll, prob, logz, nullptr, stream_);
+}
+
+int rsp_rot_estep_best(rsp_rot* p, const double* model_c, const double* sx, const double* sy,
+                       const double* dia, const double* scales, double* ll, double* prob,
+                       double* logz, int32_t* best, rsp_stream_t stream_) {
   if (!p || !model_c || !sx || !sy || !dia || !scales)
     return fail(kErrNull, "rsp_rot_estep: null argument");
   if (p->N == 0) return fail(kErrState, "rsp_rot_estep: no frames");
+  if (p->pass_flags >= 0) return fail(kErrState, "rsp_rot_estep: a pass is open");
   hipStream_t st = (hipStream_t)stream_;
   RSP_HIP(hipSetDevice(p->device));
-  const int64_t eb = elem_bytes(p->mode);
-  int64_t chunk = p->chunk;
-  if (chunk <= 0) {
-    // operand rows of a chunk within about 8 GB
-    chunk = std::max<int64_t>(128, (int64_t)(8e9 / (double)(p->Kp * eb)) / 128 * 128);
-  }
-  chunk = std::min(chunk, xterm_pad_n(p->N));
-  int rc = ensure_work(p, chunk);
+  int rc = ensure_work(p, default_chunk(p, p->N));
   if (rc) return rc;
   if (p->timing) {
-    for (auto& e : p->ev)
-      if (!e) RSP_HIP(hipEventCreate(&e));
+    if ((rc = ensure_events(p))) return rc;
     for (double& t : p->t_ms) t = 0.;
     RSP_HIP(hipEventRecord(p->ev[0], st));
   }
-  if (p->mode == kFP32) rc = launch_slices<kFP32>(p, model_c, st);
-  else if (p->mode == kSplit) rc = launch_slices<kSplit>(p, model_c, st);
-  else rc = launch_slices<kFP64>(p, model_c, st);
-  if (rc) return rc;
+  if ((rc = launch_slices(p, model_c, st))) return rc;
   if (p->timing) {
     RSP_HIP(hipEventRecord(p->ev[1], st));
     RSP_HIP(hipEventSynchronize(p->ev[1]));
@@ -1112,47 +1421,7 @@ int rsp_rot_estep(rsp_rot* p, const double* model_c, const double* sx, const dou
     RSP_HIP(hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
     p->t_ms[0] += ms;
   }
-  const int64_t Bpad = xterm_pad_n(chunk);
-  for (int64_t d0 = 0; d0 < p->N; d0 += chunk) {
-    const int64_t n = std::min(chunk, p->N - d0);
-    if (p->timing) RSP_HIP(hipEventRecord(p->ev[2], st));
-    if (p->mode == kFP32) rc = launch_ref<kFP32>(p, sx, sy, dia, d0, n, st);
-    else if (p->mode == kSplit) rc = launch_ref<kSplit>(p, sx, sy, dia, d0, n, st);
-    else rc = launch_ref<kFP64>(p, sx, sy, dia, d0, n, st);
-    if (rc) return rc;
-    if (p->timing) RSP_HIP(hipEventRecord(p->ev[3], st));
-    if (p->mode == kFP32) {
-      rc = xterm_gemm_blas32((const float*)p->d_A, (const float*)p->d_B, (float*)p->d_X, p->Mpad,
-                             n, p->Kp, p->Kp, p->Kp, p->Mpad, st);
-    } else if (p->mode == kFP64) {
-      rc = xterm_gemm_blas64((const double*)p->d_A, (const double*)p->d_B, (double*)p->d_X,
-                             p->Mpad, n, p->Kp, p->Kp, p->Kp, p->Mpad, st);
-    } else {
-      const __bf16* ah = (const __bf16*)p->d_A;
-      const __bf16* bh = (const __bf16*)p->d_B;
-      rc = xterm_gemm_split(ah, ah + p->Mpad * p->Kp, bh, bh + Bpad * p->Kp, (float*)p->d_X,
-                            p->Mpad, n, p->Kp, p->Mpad, st);
-    }
-    if (rc) return rc;
-    if (p->timing) RSP_HIP(hipEventRecord(p->ev[4], st));
-    if (p->mode == kFP64)
-      rc = launch_ll<double>(p, sx, sy, dia, scales, d0, d0 + n, ll, prob, logz, st);
-    else
-      rc = launch_ll<float>(p, sx, sy, dia, scales, d0, d0 + n, ll, prob, logz, st);
-    if (rc) return rc;
-    if (p->timing) {
-      RSP_HIP(hipEventRecord(p->ev[5], st));
-      RSP_HIP(hipEventSynchronize(p->ev[5]));
-      float a = 0.f, b = 0.f, c = 0.f;
-      RSP_HIP(hipEventElapsedTime(&a, p->ev[2], p->ev[3]));
-      RSP_HIP(hipEventElapsedTime(&b, p->ev[3], p->ev[4]));
-      RSP_HIP(hipEventElapsedTime(&c, p->ev[4], p->ev[5]));
-      p->t_ms[1] += a;
-      p->t_ms[2] += b;
-      p->t_ms[3] += c;
-      p->t_ms[4] += 1.;
-    }
-  }
+  if ((rc = run_estep(p, resident(p), sx, sy, dia, scales, ll, prob, logz, best, st))) return rc;
   RSP_HIP(hipStreamSynchronize(st));
   return kOk;
 }
@@ -1161,18 +1430,15 @@ int rsp_rot_merge(rsp_rot* p, const double* prob, const double* scales, double* 
                   double* acc_out, double* weights_out, rsp_stream_t stream_) {
   if (!p || !prob || !scales || !model_out) return fail(kErrNull, "rsp_rot_merge: null argument");
   if (p->N == 0) return fail(kErrState, "rsp_rot_merge: no frames");
+  if (p->pass_flags >= 0) return fail(kErrState, "rsp_rot_merge: a pass is open");
   hipStream_t st = (hipStream_t)stream_;
   RSP_HIP(hipSetDevice(p->device));
   int rc;
-  if (!p->d_VT && (rc = dalloc(&p->d_VT, (int64_t)p->K * p->Rs))) return rc;
-  if ((rc = launch_wsum(p, prob, scales, st))) return rc;
-  hipLaunchKernelGGL(k_merge_view, dim3((unsigned)p->K), dim3(256), 0, st, p->d_po, p->d_pd,
-                     p->d_pk, prob, p->R, p->Rs, p->d_VT);
-  RSP_LAUNCH_CHECK("k_merge_view");
-  hipLaunchKernelGGL(k_merge_model, dim3((unsigned)ceil_div(p->npix, 256)), dim3(256), 0, st,
-                     p->size, p->R, p->Rs, p->d_rcos, p->d_rsin, p->d_compact_of, p->d_VT,
-                     p->d_wsum, model_out, acc_out, weights_out);
-  RSP_LAUNCH_CHECK("k_merge_model");
+  const Csr f = resident(p);
+  if ((rc = run_wsum(p, prob, scales, p->N, false, st)) ||
+      (rc = run_merge_view(p, f, prob, false, st)) ||
+      (rc = run_merge_model(p, model_out, acc_out, weights_out, st)))
+    return rc;
   RSP_HIP(hipStreamSynchronize(st));
   return kOk;
 }
@@ -1185,45 +1451,130 @@ int rsp_rot_adjoint(rsp_rot* p, const double* model_c, const double* sx, const d
       !g_dia)
     return fail(kErrNull, "rsp_rot_adjoint: null argument");
   if (p->N == 0) return fail(kErrState, "rsp_rot_adjoint: no frames");
+  if (p->pass_flags >= 0) return fail(kErrState, "rsp_rot_adjoint: a pass is open");
   hipStream_t st = (hipStream_t)stream_;
   RSP_HIP(hipSetDevice(p->device));
-  const size_t shmem =
-      sizeof(double) * (size_t)((2 * p->nrad + 1) & ~1) + 2 * sizeof(double2) * (size_t)p->size;
-  int lds_max = 0;
-  RSP_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p->device));
-  if (shmem + 64 > (size_t)lds_max)
-    return fail(kErrSize, "rsp_rot_adjoint: reference tables exceed the LDS of a work-group");
-  int64_t chunk = p->chunk > 0 ? p->chunk : 4096;
-  chunk = std::min(chunk, p->N);
-  int rc = ensure_adj(p, chunk);
+  int rc = check_adj_lds(p);
   if (rc) return rc;
+  const int64_t chunk = std::min(p->chunk > 0 ? p->chunk : 4096, p->N);
+  if ((rc = ensure_adj(p, chunk))) return rc;
   if ((rc = launch_slices_to<kFP64>(p, model_c, p->d_aA, p->d_aOT, p->d_apart, p->d_anormO, st)))
     return rc;
-  if ((rc = launch_wsum(p, prob, scales, st))) return rc;
-  for (int64_t d0 = 0; d0 < p->N; d0 += chunk) {
-    const int64_t n = std::min(chunk, p->N - d0);
-    if ((rc = launch_ref_to<kFP64>(p, sx, sy, dia, d0, n, p->d_aB, chunk, p->d_anormR, st)))
+  const Csr f = resident(p);
+  if ((rc = run_wsum(p, prob, scales, p->N, false, st)) ||
+      (rc = run_adj_frames(p, f, prob, sx, sy, dia, scales, g_sx, g_sy, g_dia, true, true, true,
+                           st)) ||
+      (rc = run_adj_pixel(p, f, prob, sx, sy, dia, scales, false, st)) ||
+      (rc = run_adj_finish(p, g_model, st)))
+    return rc;
+  RSP_HIP(hipStreamSynchronize(st));
+  return kOk;
+}
+
+int rsp_rot_pass_begin(rsp_rot* p, const double* model_c, int flags, int64_t max_chunk,
+                       rsp_stream_t stream_) {
+  if (!p || !model_c) return fail(kErrNull, "rsp_rot_pass_begin: null argument");
+  if (p->pass_flags >= 0) return fail(kErrState, "rsp_rot_pass_begin: a pass is open");
+  if (flags < 0 || flags > 7) return fail(kErrSize, "rsp_rot_pass_begin: flags");
+  if (max_chunk < 1 || max_chunk > 0x7fffffff)
+    return fail(kErrSize, "rsp_rot_pass_begin: max_chunk");
+  hipStream_t st = (hipStream_t)stream_;
+  RSP_HIP(hipSetDevice(p->device));
+  int rc = ensure_work(p, default_chunk(p, max_chunk));
+  if (rc) return rc;
+  if ((rc = launch_slices(p, model_c, st))) return rc;
+  if (flags & (kPassFrameGrad | kPassModelGrad)) {
+    if ((rc = check_adj_lds(p))) return rc;
+    if ((rc = ensure_adj(p, std::min<int64_t>(p->chunk > 0 ? p->chunk : 4096, max_chunk))))
       return rc;
-    hipLaunchKernelGGL(k_scaled_prob, dim3((unsigned)ceil_div(n * p->R, 256)), dim3(256), 0, st,
-                       prob, scales, d0, n, p->R, p->d_aQ);
-    RSP_LAUNCH_CHECK("k_scaled_prob");
-    // GT (R x Kp, column-major) += Q^T B : GT[k * R + r] = sum_d Q[d, r] B[d, k]
-    if ((rc = xterm_dgemm(false, true, p->R, p->Kp, n, 1., p->d_aQ, p->R, p->d_aB, p->Kp,
-                          d0 == 0 ? 0. : 1., p->d_aGT, p->R, st)))
-      return rc;
-    // Obar (n x Kp, row-major) = P A : Obar[d * Kp + k] = sum_r P[d, r] A[r, k]
-    if ((rc = xterm_dgemm(false, false, p->Kp, n, p->R, 1., p->d_aA, p->Kp, prob + d0 * p->R,
-                          p->R, 0., p->d_aObar, p->Kp, st)))
-      return rc;
-    if ((rc = launch_adj_frame(p, shmem, prob, sx, sy, dia, scales, d0, n, g_sx, g_sy, g_dia,
-                               st)))
+    if ((rc = launch_slices_to<kFP64>(p, model_c, p->d_aA, p->d_aOT, p->d_apart, p->d_anormO,
+                                      st)))
       return rc;
   }
-  if ((rc = launch_adj_pixel(p, prob, sx, sy, dia, scales, st))) return rc;
-  hipLaunchKernelGGL(k_adj_model, dim3((unsigned)ceil_div(p->npix, 256)), dim3(256), 0, st,
-                     p->size, p->R, p->Rs, p->d_rcos, p->d_rsin, p->d_compact_of, p->d_aGO,
-                     g_model);
-  RSP_LAUNCH_CHECK("k_adj_model");
+  if (flags & (kPassMerge | kPassModelGrad)) {
+    if ((rc = ensure_merge(p, ceil_div(max_chunk, 1024)))) return rc;
+    if (p->pc_alloc < max_chunk) {
+      if ((rc = dalloc(&p->d_Pc, max_chunk * p->H))) return rc;
+      p->pc_alloc = max_chunk;
+    }
+  } else if (flags & kPassFrameGrad) {
+    if (p->pc_alloc < max_chunk) {
+      if ((rc = dalloc(&p->d_Pc, max_chunk * p->H))) return rc;
+      p->pc_alloc = max_chunk;
+    }
+  }
+  if (p->timing)
+    for (double& t : p->t_ms) t = 0.;
+  p->pass_flags = flags;
+  p->pass_max = max_chunk;
+  p->pass_first = true;
+  return kOk;
+}
+
+int rsp_rot_pass_chunk(rsp_rot* p, int64_t n, const int64_t* fo, const int32_t* fj,
+                       const float* fk, const int64_t* po, const int32_t* pd, const float* pk,
+                       const double* sx, const double* sy, const double* dia,
+                       const double* scales, double* ll, double* prob, double* logz,
+                       int32_t* best, double* g_sx, double* g_sy, double* g_dia,
+                       rsp_stream_t stream_) {
+  if (!p) return fail(kErrNull, "rsp_rot_pass_chunk: null plan");
+  if (p->pass_flags < 0) return fail(kErrState, "rsp_rot_pass_chunk: no open pass");
+  if (n < 1 || n > p->pass_max) return fail(kErrSize, "rsp_rot_pass_chunk: n > max_chunk");
+  const int flags = p->pass_flags;
+  if (!fo || !fj || !fk || !sx || !sy || !dia || !scales)
+    return fail(kErrNull, "rsp_rot_pass_chunk: null argument");
+  if ((flags & (kPassMerge | kPassModelGrad)) && (!po || !pd || !pk))
+    return fail(kErrNull, "rsp_rot_pass_chunk: pixel-major lists needed");
+  if ((flags & kPassFrameGrad) && (!g_sx || !g_sy || !g_dia))
+    return fail(kErrNull, "rsp_rot_pass_chunk: frame gradient outputs needed");
+  hipStream_t st = (hipStream_t)stream_;
+  RSP_HIP(hipSetDevice(p->device));
+  Csr f;
+  f.fo = fo;
+  f.fj = fj;
+  f.fk = fk;
+  f.po = po;
+  f.pd = pd;
+  f.pk = pk;
+  f.n = n;
+  double* P = prob ? prob : (flags ? p->d_Pc : nullptr);
+  int rc;
+  if ((rc = run_estep(p, f, sx, sy, dia, scales, ll, P, logz, best, st))) return rc;
+  const bool acc = !p->pass_first;
+  if (flags & (kPassMerge | kPassModelGrad))
+    if ((rc = run_wsum(p, P, scales, n, acc, st))) return rc;
+  if (flags & kPassMerge)
+    if ((rc = run_merge_view(p, f, P, acc, st))) return rc;
+  if (flags & (kPassFrameGrad | kPassModelGrad))
+    if ((rc = run_adj_frames(p, f, P, sx, sy, dia, scales, g_sx, g_sy, g_dia,
+                             (flags & kPassFrameGrad) != 0, (flags & kPassModelGrad) != 0,
+                             p->pass_first, st)))
+      return rc;
+  if (flags & kPassModelGrad)
+    if ((rc = run_adj_pixel(p, f, P, sx, sy, dia, scales, acc, st))) return rc;
+  p->pass_first = false;
+  return kOk;
+}
+
+int rsp_rot_pass_end(rsp_rot* p, double* model_out, double* acc_out, double* weights_out,
+                     double* g_model, rsp_stream_t stream_) {
+  if (!p) return fail(kErrNull, "rsp_rot_pass_end: null plan");
+  if (p->pass_flags < 0) return fail(kErrState, "rsp_rot_pass_end: no open pass");
+  const int flags = p->pass_flags;
+  p->pass_flags = -1;
+  hipStream_t st = (hipStream_t)stream_;
+  RSP_HIP(hipSetDevice(p->device));
+  if (p->pass_first) {
+    RSP_HIP(hipStreamSynchronize(st));
+    if (flags & (kPassMerge | kPassModelGrad))
+      return fail(kErrState, "rsp_rot_pass_end: the pass had no chunks");
+    return kOk;
+  }
+  int rc;
+  if ((flags & kPassMerge) && model_out)
+    if ((rc = run_merge_model(p, model_out, acc_out, weights_out, st))) return rc;
+  if ((flags & kPassModelGrad) && g_model)
+    if ((rc = run_adj_finish(p, g_model, st))) return rc;
   RSP_HIP(hipStreamSynchronize(st));
   return kOk;
 }
@@ -1243,6 +1594,9 @@ int rsp_rot_info(const rsp_rot* p, int64_t* out) {
   out[4] = p->chunk_alloc;
   out[5] = p->nnz;
   out[6] = p->nrad;
+  out[7] = p->C;
+  out[8] = p->H;
+  out[9] = p->Hs;
   return kOk;
 }
 
@@ -1251,11 +1605,20 @@ int rsp_rot_destroy(rsp_rot* p) {
   (void)hipSetDevice(p->device);
   free_work(p);
   free_adj(p);
-  void* ptrs[] = {p->d_rcos, p->d_rsin, p->d_rad, p->d_scored, p->d_jrad, p->d_compact_of,
-                  p->d_fo, p->d_po, p->d_fj, p->d_pd, p->d_fk, p->d_pk, p->d_VT, p->d_wsum,
-                  p->d_wpart};
-  for (void* q : ptrs)
-    if (q) (void)hipFree(q);
+  free_merge(p);
+  dfree(p->d_Pc);
+  dfree(p->d_rcos);
+  dfree(p->d_rsin);
+  dfree(p->d_rad);
+  dfree(p->d_scored);
+  dfree(p->d_jrad);
+  dfree(p->d_compact_of);
+  dfree(p->d_fo);
+  dfree(p->d_po);
+  dfree(p->d_fj);
+  dfree(p->d_pd);
+  dfree(p->d_fk);
+  dfree(p->d_pk);
   for (hipEvent_t e : p->ev)
     if (e) (void)hipEventDestroy(e);
   delete p;

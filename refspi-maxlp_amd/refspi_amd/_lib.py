@@ -47,7 +47,12 @@ _SIGS = {
     'rsp_rot_set_option': (C.c_int, [vp, C.c_int, C.c_int64]),
     'rsp_rot_estep': (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     'rsp_rot_merge': (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    'rsp_rot_estep_best': (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     'rsp_rot_adjoint': (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    'rsp_rot_pack': (C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    'rsp_rot_pass_begin': (C.c_int, [vp, vp, C.c_int, C.c_int64, vp]),
+    'rsp_rot_pass_chunk': (C.c_int, [vp, C.c_int64] + [vp] * 18),
+    'rsp_rot_pass_end': (C.c_int, [vp, vp, vp, vp, vp, vp]),
     'rsp_rot_timing': (C.c_int, [vp, vp]),
     'rsp_rot_info': (C.c_int, [vp, i64p]),
     'rsp_rot_destroy': (C.c_int, [vp]),

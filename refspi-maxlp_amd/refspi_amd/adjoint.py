@@ -24,9 +24,10 @@ __all__ = ['RotLoss', 'rot_loss']
 
 
 class RotLoss(torch.autograd.Function):
-    """Inputs: model (npix, 2) real (re, im; FP32 or FP64), sx, sy, dia (N,) real; the
-    plan (:class:`~refspi_amd.rotem.RotEM`) and per-frame scales (N,) or None are
-    constants.  Output: the FP64 scalar loss."""
+    """Inputs: model (npix, 2) or, for several classes, (C, npix, 2) real (re, im; FP32 or
+    FP64), sx, sy, dia (N,) real; the plan (:class:`~refspi_amd.rotem.RotEM`) and per-frame
+    scales (N,) or None are constants.  Output: the FP64 scalar loss
+    -sum_d log((1/H) sum_h exp LL[d, h]) over the plan's H = C * R hypotheses."""
 
     @staticmethod
     def forward(ctx, model, sx, sy, dia, plan, scales):
@@ -35,7 +36,7 @@ class RotLoss(torch.autograd.Function):
         a = [v.detach().to(f64).contiguous() for v in (sx, sy, dia)]
         sc = None if scales is None else torch.as_tensor(scales).to(m.device, f64)
         _, prob, logz = plan.estep_device(m, *a, sc, want_ll=False, want_prob=True)
-        loss = -(logz - math.log(plan.num_rot)).sum()
+        loss = -(logz - math.log(plan.num_hyp)).sum()
         ctx.plan = plan
         ctx.scales = sc
         ctx.dtypes = (model.dtype, sx.dtype, sy.dtype, dia.dtype)

@@ -42,6 +42,7 @@ class SynthConfig:
     fluence: str = 'constant'          # or 'lognormal'
     fluence_sigma: float = 0.5
     seed: int = 0
+    class_weights: tuple = ()          # () = one class; else the mixing fractions
 
     @property
     def box(self):
@@ -84,6 +85,24 @@ def _blobs(cfg: SynthConfig, rng):
     return cen, amp
 
 
+def _class_blobs(cfg: SynthConfig, rng):
+    """Blobs of every class: class 0 from the main stream ``rng`` (so one class equals the
+    single-class generator), class c > 0 from its own stream seeded (seed, c)."""
+    out = [_blobs(cfg, rng)]
+    for c in range(1, max(1, len(cfg.class_weights))):
+        out.append(_blobs(cfg, np.random.default_rng([cfg.seed, c])))
+    return out
+
+
+def _class_labels(cfg: SynthConfig, n):
+    """Per-frame class (int) drawn with the mixing fractions from stream (seed, 'class')."""
+    if len(cfg.class_weights) <= 1:
+        return np.zeros(n, dtype='i4')
+    w = np.asarray(cfg.class_weights, dtype='f8')
+    rng = np.random.default_rng([cfg.seed, 0x636c])
+    return rng.choice(len(w), size=n, p=w / w.sum()).astype('i4')
+
+
 def _grid(size):
     x = np.arange(size, dtype='f8') - size // 2
     return x
@@ -109,11 +128,36 @@ def detector_mask(size, beamstop):
     return ((rad >= beamstop) & (rad < size // 2)).astype('f8')
 
 
+def _object_ft_of(cfg: SynthConfig, cen, amp):
+    """Unrotated transform of blobs (cen, amp) on the q-grid, (size, size) complex."""
+    q = _grid(cfg.size) / cfg.box
+    qq = q[:, None] ** 2 + q[None, :] ** 2
+    env = 2 * np.pi * cfg.sigma ** 2 * np.exp(-2 * np.pi ** 2 * cfg.sigma ** 2 * qq)
+    A = np.exp(-2j * np.pi * q[:, None] * cen[None, :, 0]) * amp[None, :]
+    B = np.exp(-2j * np.pi * q[:, None] * cen[None, :, 1])
+    return env * (A @ B.T)
+
+
+def _amplitudes(cfg: SynthConfig, blobs):
+    """Global amplitudes of the forward model: alpha_c scales class c's object transform
+    so that it carries mean_count / (1 + ref_factor) photons over the detector mask, and
+    rel scales the reference sphere so that one of the mean diameter carries ref_factor
+    times that (one constant for all frames, as the reference generator's rel_scale,
+    make_data.py:312)."""
+    dmask = detector_mask(cfg.size, cfg.beamstop)
+    pobj = cfg.mean_count / (1. + cfg.ref_factor)
+    alpha = np.array([np.sqrt(pobj / (np.abs(_object_ft_of(cfg, c, a)) ** 2 * dmask).sum())
+                      for c, a in blobs])
+    sph = _reference_ft(cfg, np.array([cfg.dia_mean]), np.zeros(1), np.zeros(1))[0]
+    rel = np.sqrt(cfg.ref_factor * pobj / (np.abs(sph) ** 2 * dmask).sum())
+    return alpha, float(rel)
+
+
 def make_frames(cfg: SynthConfig, batch: int = 256) -> SparseFrames:
     """NumPy generator (deterministic for a given seed)."""
     rng = np.random.default_rng(cfg.seed)
     size, n = cfg.size, cfg.num_data
-    cen, amp = _blobs(cfg, rng)
+    blobs = _class_blobs(cfg, rng)
     angles = 2 * np.pi * rng.random(n)
     dia = cfg.dia_mean + cfg.dia_sd * rng.standard_normal(n)
     shifts = rng.uniform(-cfg.shift_max, cfg.shift_max, (n, 2))
@@ -125,22 +169,29 @@ def make_frames(cfg: SynthConfig, batch: int = 256) -> SparseFrames:
     qq = q[:, None] ** 2 + q[None, :] ** 2
     env = 2 * np.pi * cfg.sigma ** 2 * np.exp(-2 * np.pi ** 2 * cfg.sigma ** 2 * qq)
     dmask = detector_mask(size, cfg.beamstop)
+    labels = _class_labels(cfg, n)
+    cen_all = np.stack([b[0] for b in blobs])          # (C, n_blobs, 2)
+    amp_all = np.stack([b[1] for b in blobs])          # (C, n_blobs)
+    alpha, rel = _amplitudes(cfg, blobs)
+    scale = np.empty(n)
     ones, multi, po, pm, cm = [], [], [], [], []
     for b0 in range(0, n, batch):
         b1 = min(n, b0 + batch)
         ca, sa = np.cos(angles[b0:b1]), np.sin(angles[b0:b1])
+        cen = cen_all[labels[b0:b1]]                   # (b, n_blobs, 2)
+        amp = amp_all[labels[b0:b1]]
         # R^T c for each frame: centres rotated by -angle
-        u = ca[:, None] * cen[None, :, 0] + sa[:, None] * cen[None, :, 1]
-        v = -sa[:, None] * cen[None, :, 0] + ca[:, None] * cen[None, :, 1]
-        A = np.exp(-2j * np.pi * q[None, :, None] * u[:, None, :]) * amp[None, None, :]
+        u = ca[:, None] * cen[:, :, 0] + sa[:, None] * cen[:, :, 1]
+        v = -sa[:, None] * cen[:, :, 0] + ca[:, None] * cen[:, :, 1]
+        A = np.exp(-2j * np.pi * q[None, :, None] * u[:, None, :]) * amp[:, None, :]
         B = np.exp(-2j * np.pi * q[None, :, None] * v[:, None, :])
-        fobj = env[None] * np.matmul(A, B.transpose(0, 2, 1))
-        fref = _reference_ft(cfg, dia[b0:b1], shifts[b0:b1, 0], shifts[b0:b1, 1])
-        pobj = (np.abs(fobj) ** 2 * dmask).sum((1, 2))
-        pref = (np.abs(fref) ** 2 * dmask).sum((1, 2))
-        rel = np.sqrt(cfg.ref_factor * pobj / pref)
-        inten = np.abs(fobj + rel[:, None, None] * fref) ** 2 * dmask
-        inten *= (cfg.mean_count * fl[b0:b1] / inten.sum((1, 2)))[:, None, None]
+        fobj = alpha[labels[b0:b1]][:, None, None] * env[None] * \
+            np.matmul(A, B.transpose(0, 2, 1))
+        fref = rel * _reference_ft(cfg, dia[b0:b1], shifts[b0:b1, 0], shifts[b0:b1, 1])
+        inten = np.abs(fobj + fref) ** 2 * dmask
+        # each frame carries mean_count * fluence photons (make_data.py:319)
+        scale[b0:b1] = cfg.mean_count * fl[b0:b1] / inten.sum((1, 2))
+        inten *= scale[b0:b1, None, None]
         frames = rng.poisson(inten).reshape(b1 - b0, -1)
         for f in frames:
             o = np.nonzero(f == 1)[0]
@@ -154,7 +205,8 @@ def make_frames(cfg: SynthConfig, batch: int = 256) -> SparseFrames:
     return SparseFrames(size * size, np.array(ones, 'i4'), np.array(multi, 'i4'), cat(po),
                         cat(pm), cat(cm), None,
                         dict(angles=angles, diameters=dia, shifts=shifts, fluence=fl,
-                             blob_centres=cen, blob_amps=amp))
+                             scale=scale, blob_centres=cen_all[0], blob_amps=amp_all[0],
+                             classes=labels, alpha=alpha, rel=rel))
 
 
 def make_frames_torch(cfg: SynthConfig, device='cuda', batch: int = 1024):
@@ -169,9 +221,13 @@ def make_frames_torch(cfg: SynthConfig, device='cuda', batch: int = 1024):
     size, n = cfg.size, cfg.num_data
     f32 = torch.float32
     rng = np.random.default_rng(cfg.seed)
-    cen, amp = _blobs(cfg, rng)
-    cen = torch.tensor(cen, dtype=f32, device=device)
-    amp = torch.tensor(amp, dtype=f32, device=device)
+    blobs = _class_blobs(cfg, rng)
+    cen_all = torch.tensor(np.stack([b[0] for b in blobs]), dtype=f32, device=device)
+    amp_all = torch.tensor(np.stack([b[1] for b in blobs]), dtype=f32, device=device)
+    labels = torch.as_tensor(_class_labels(cfg, n), device=device, dtype=torch.int64)
+    alpha_np, rel = _amplitudes(cfg, blobs)
+    alpha = torch.tensor(alpha_np, dtype=f32, device=device)
+    scale_l = []
     angles = 2 * np.pi * torch.rand(n, generator=gen, device=device, dtype=torch.float64)
     dia = cfg.dia_mean + cfg.dia_sd * torch.randn(n, generator=gen, device=device,
                                                   dtype=torch.float64)
@@ -195,27 +251,29 @@ def make_frames_torch(cfg: SynthConfig, device='cuda', batch: int = 1024):
         b1 = min(n, b0 + batch)
         a = angles[b0:b1].to(f32)
         ca, sa = torch.cos(a), torch.sin(a)
-        u = ca[:, None] * cen[None, :, 0] + sa[:, None] * cen[None, :, 1]
-        v = -sa[:, None] * cen[None, :, 0] + ca[:, None] * cen[None, :, 1]
+        cen = cen_all[labels[b0:b1]]
+        amp = amp_all[labels[b0:b1]][:, None, :]
+        u = ca[:, None] * cen[:, :, 0] + sa[:, None] * cen[:, :, 1]
+        v = -sa[:, None] * cen[:, :, 0] + ca[:, None] * cen[:, :, 1]
         pa = -2 * np.pi * q[None, :, None] * u[:, None, :]
         pb = -2 * np.pi * q[None, :, None] * v[:, None, :]
         Ar, Ai = torch.cos(pa) * amp, torch.sin(pa) * amp
         Br = torch.cos(pb).transpose(1, 2).contiguous()
         Bi = torch.sin(pb).transpose(1, 2).contiguous()
-        fr = env * (torch.bmm(Ar, Br) - torch.bmm(Ai, Bi))
-        fi = env * (torch.bmm(Ar, Bi) + torch.bmm(Ai, Br))
+        al = alpha[labels[b0:b1]][:, None, None] * env
+        fr = al * (torch.bmm(Ar, Br) - torch.bmm(Ai, Bi))
+        fi = al * (torch.bmm(Ar, Bi) + torch.bmm(Ai, Br))
         d = dia[b0:b1].to(f32)[:, None, None]
         s = rad[None] * np.pi * d / size
         s = torch.where(s == 0, torch.full_like(s, 1.e-5), s)
         sph = (torch.sin(s) - s * torch.cos(s)) / (s * s * s)
         sh = shifts[b0:b1].to(f32)
         ph = 2 * np.pi * (xx[None] * sh[:, 0, None, None] + yy[None] * sh[:, 1, None, None]) / size
-        rr, ri = sph * torch.cos(ph), sph * torch.sin(ph)
-        pobj = ((fr * fr + fi * fi) * dmask).sum((1, 2))
-        pref = ((rr * rr + ri * ri) * dmask).sum((1, 2))
-        rel = torch.sqrt(cfg.ref_factor * pobj / pref)[:, None, None]
-        inten = ((fr + rel * rr) ** 2 + (fi + rel * ri) ** 2) * dmask
-        inten = inten * (cfg.mean_count * fl[b0:b1].to(f32) / inten.sum((1, 2)))[:, None, None]
+        rr, ri = rel * sph * torch.cos(ph), rel * sph * torch.sin(ph)
+        inten = ((fr + rr) ** 2 + (fi + ri) ** 2) * dmask
+        sc = cfg.mean_count * fl[b0:b1].to(f32) / inten.sum((1, 2))
+        scale_l.append(sc.to(torch.float64))
+        inten = inten * sc[:, None, None]
         k = torch.poisson(inten, generator=gen).reshape(b1 - b0, -1).to(torch.int32)
         one = k == 1
         mul = k > 1
@@ -228,42 +286,30 @@ def make_frames_torch(cfg: SynthConfig, device='cuda', batch: int = 1024):
     return dict(num_pix=size * size, ones=torch.cat(ones_l), multi=torch.cat(multi_l),
                 place_ones=torch.cat(po_l), place_multi=torch.cat(pm_l),
                 count_multi=torch.cat(cm_l), angles=angles, diameters=dia, shifts=shifts,
-                fluence=fl)
+                fluence=fl, scale=torch.cat(scale_l), classes=labels.to(torch.int32),
+                alpha=alpha_np, rel=rel)
 
 
 PRESETS = {'cpu64': CONFIG_CPU64, '185': CONFIG_185, '256': CONFIG_256}
 
 
-def object_ft(cfg: SynthConfig):
-    """The generator's unrotated object transform on the q-grid (complex, size*size,
-    flat index x*size + y), from the same blob draw as :func:`make_frames` /
-    :func:`make_frames_torch` for ``cfg.seed``."""
+def object_ft(cfg: SynthConfig, cls: int = 0):
+    """The generator's unrotated transform of object class ``cls`` on the q-grid
+    (complex, size*size, flat index x*size + y), before its amplitude factor, from the same
+    blob draws as :func:`make_frames` / :func:`make_frames_torch` for ``cfg.seed``."""
     rng = np.random.default_rng(cfg.seed)
-    cen, amp = _blobs(cfg, rng)
-    q = _grid(cfg.size) / cfg.box
-    qq = q[:, None] ** 2 + q[None, :] ** 2
-    env = 2 * np.pi * cfg.sigma ** 2 * np.exp(-2 * np.pi ** 2 * cfg.sigma ** 2 * qq)
-    A = np.exp(-2j * np.pi * q[:, None] * cen[None, :, 0]) * amp[None, :]
-    B = np.exp(-2j * np.pi * q[:, None] * cen[None, :, 1])
-    return (env * (A @ B.T)).ravel()
+    cen, amp = _class_blobs(cfg, rng)[cls]
+    return _object_ft_of(cfg, cen, amp).ravel()
 
 
-def model_scales(cfg: SynthConfig, pix, dia=None):
-    """(object amplitude factor, reference rel_scale) that give a model |alpha F|^2 with
-    mean_count / (1 + ref_factor) photons over the pixels ``pix`` and a reference sphere of
-    diameter ``dia`` (default dia_mean) with ref_factor times that: the photon budget of the
-    generator, for rotation-form workloads whose model is the object transform."""
-    f = object_ft(cfg)
-    dia = cfg.dia_mean if dia is None else dia
-    x, y = pix // cfg.size, pix % cfg.size
-    c = cfg.size // 2
-    s = np.hypot(x - c, y - c) * np.pi * dia / cfg.size
-    s = np.where(s == 0, 1.e-5, s)
-    sph = (np.sin(s) - s * np.cos(s)) / s ** 3
-    pobj = cfg.mean_count / (1. + cfg.ref_factor)
-    alpha = np.sqrt(pobj / (np.abs(f[pix]) ** 2).sum())
-    rel = np.sqrt(cfg.ref_factor * pobj / (sph ** 2).sum())
-    return alpha, rel
+def model_scales(cfg: SynthConfig, cls: int = 0):
+    """(alpha_cls, rel): the generator's amplitude of class ``cls``'s object transform and
+    of the reference sphere.  With model = alpha * object_ft(cfg, cls), rel_scale = rel and
+    per-frame scales = truth['scale'], the rotation form's W is the generator's intensity
+    (up to the interpolation of the rotated model)."""
+    rng = np.random.default_rng(cfg.seed)
+    alpha, rel = _amplitudes(cfg, _class_blobs(cfg, rng))
+    return float(alpha[cls]), rel
 
 
 def write_photons_file(path, cfg: SynthConfig):

@@ -192,3 +192,63 @@ def test_loss_grad_central_differences(setup):
         assert abs(gsx[d] - fsx) < 1e-5 * max(1., abs(fsx))
         assert abs(gsy[d] - fsy) < 1e-5 * max(1., abs(fsy))
         assert abs(gdia[d] - fdd) < 1e-5 * max(1., abs(fdd))
+
+
+def test_multi_class_reduces_to_single(setup):
+    model, sx, sy, dia, pix, K, scales = setup
+    R = 3
+    LL1, _ = O.rot_loglik(model, SIZE, R, K, sx, sy, dia, scales, 1.2)
+    LL = O.rot_loglik_multi(model[None], SIZE, R, K, sx, sy, dia, scales, 1.2)
+    assert np.array_equal(LL, LL1)
+    a = O.loss_grad(model, SIZE, R, K, sx, sy, dia, scales, 1.2)
+    b = O.loss_grad_multi(model[None], SIZE, R, K, sx, sy, dia, scales, 1.2)
+    assert a[0] == b[0]
+    np.testing.assert_allclose(b[1][0], a[1], rtol=1e-13, atol=1e-13 * np.abs(a[1]).max())
+    for x, y in zip(a[2:], b[2:]):
+        np.testing.assert_allclose(y, x, rtol=1e-13, atol=1e-13 * np.abs(x).max())
+
+
+def test_multi_class_identical_models_split_probability(setup):
+    """Two identical classes: every hypothesis appears twice, so P halves, the loss is
+    unchanged (the prior 1/(C R) compensates) and each class gets half the model gradient."""
+    model, sx, sy, dia, pix, K, scales = setup
+    R = 3
+    LL = O.rot_loglik_multi(np.stack([model, model]), SIZE, R, K, sx, sy, dia, scales, 1.2)
+    P, _ = O.lse(LL)
+    P1, _ = O.lse(LL[:, :R])
+    np.testing.assert_allclose(P[:, :R], 0.5 * P1, rtol=1e-13)
+    a = O.loss_grad(model, SIZE, R, K, sx, sy, dia, scales, 1.2)
+    b = O.loss_grad_multi(np.stack([model, model]), SIZE, R, K, sx, sy, dia, scales, 1.2)
+    assert b[0] == pytest.approx(a[0], rel=1e-13)
+    for c in range(2):
+        np.testing.assert_allclose(b[1][c], 0.5 * a[1], rtol=1e-12,
+                                   atol=1e-13 * np.abs(a[1]).max())
+    m1, _, _ = O.merge_model(P1, K, scales, SIZE, R)
+    mm, _, _ = O.merge_model_multi(P, K, scales, SIZE, R)
+    np.testing.assert_allclose(mm[0], m1, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(mm[1], m1, rtol=1e-12, atol=1e-14)
+
+
+def test_multi_class_loss_grad_central_differences(setup):
+    model, sx, sy, dia, pix, K, scales = setup
+    R = 2
+    rng = np.random.default_rng(9)
+    models = np.stack([model, 0.3 * (rng.random(SIZE * SIZE) + 1j * rng.random(SIZE * SIZE))])
+    f = lambda ms, a, b, c: O.total_loss(O.rot_loglik_multi(ms, SIZE, R, K, a, b, c, scales, 1.5))
+    loss, gm, gsx, gsy, gdia = O.loss_grad_multi(models, SIZE, R, K, sx, sy, dia, scales, 1.5)
+    assert loss == pytest.approx(f(models, sx, sy, dia), rel=1e-14)
+    h = 1e-6
+    for c in range(2):
+        for t in rng.choice(pix, 3, replace=False):
+            e = np.zeros_like(models)
+            e[c, t] = h
+            fr = (f(models + e, sx, sy, dia) - f(models - e, sx, sy, dia)) / (2 * h)
+            fi = (f(models + 1j * e, sx, sy, dia) - f(models - 1j * e, sx, sy, dia)) / (2 * h)
+            assert abs(gm[c, t].real - fr) < 1e-5 * max(1., abs(fr))
+            assert abs(gm[c, t].imag - fi) < 1e-5 * max(1., abs(fi))
+    e = np.zeros(len(sx))
+    e[2] = h
+    fdd = (f(models, sx, sy, dia + e) - f(models, sx, sy, dia - e)) / (2 * h)
+    assert abs(gdia[2] - fdd) < 1e-5 * max(1., abs(fdd))
+    fsx = (f(models, sx + e, sy, dia) - f(models, sx - e, sy, dia)) / (2 * h)
+    assert abs(gsx[2] - fsx) < 1e-5 * max(1., abs(fsx))
