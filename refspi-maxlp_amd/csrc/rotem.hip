@@ -145,10 +145,14 @@ __global__ __launch_bounds__(256) void k_rot_slices(
     const int jj = blockIdx.x * 64 + q, h = h0 + hl;
     if (jj >= K || h >= Hs) continue;
     const double2 v = tile[hl][q];
-    if constexpr (MODE == kFP64)
+    if constexpr (MODE == kFP64) {
       ((double2*)OT)[(int64_t)jj * Hs + h] = v;
-    else
-      ((float2*)OT)[(int64_t)jj * Hs + h] = make_float2((float)v.x, (float)v.y);
+    } else {
+      // FP32 hi + lo: O = hi + lo to about 2^-48 relative (see k_rot_ll)
+      const float hr = (float)v.x, hi = (float)v.y;
+      ((float4*)OT)[(int64_t)jj * Hs + h] =
+          make_float4(hr, hi, (float)(v.x - (double)hr), (float)(v.y - (double)hi));
+    }
   }
 }
 
@@ -258,20 +262,23 @@ __device__ __forceinline__ T log_floor(T w) {
     return w >= 1.e-20 ? log(w) : -1000.;
 }
 
-// One wave per frame of [d0, d1): LL, P, log Z, argmax.  Hypotheses h = lane + 64 t.
+// One wave per frame of [d0, d1): LL, P, log Z, argmax.  Hypotheses h = lane + 64 t.  The
+// cross term X[d - d0, h] is the sum of nxs slices xstride elements apart.
 template <typename T, int NT>
 __global__ __launch_bounds__(256) void k_rot_ll(
     int size, int H, int Hs, double rel, const int64_t* __restrict__ fo,
     const int32_t* __restrict__ fj, const float* __restrict__ fk,
     const int32_t* __restrict__ scored, const void* __restrict__ OT_,
     const double* __restrict__ normO, const double* __restrict__ normR,
-    const void* __restrict__ X_, int64_t ldx, const double* __restrict__ sx,
-    const double* __restrict__ sy, const double* __restrict__ dia,
-    const double* __restrict__ scales, int64_t d0, int64_t d1, double* __restrict__ ll,
-    double* __restrict__ prob, double* __restrict__ logz, int32_t* __restrict__ best) {
-  typedef typename std::conditional<sizeof(T) == 4, float2, double2>::type T2;
+    const void* __restrict__ X_, int64_t ldx, int nxs, int64_t xstride,
+    const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ dia, const double* __restrict__ scales, int64_t d0, int64_t d1,
+    double* __restrict__ ll, double* __restrict__ prob, double* __restrict__ logz,
+    int32_t* __restrict__ best) {
+  // OT entries: FP64 (re, im), or FP32 (re_hi, im_hi, re_lo, im_lo)
+  typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type T2;
   __shared__ int32_t s_j[4][64];
-  __shared__ T s_k[4][64], s_rr[4][64], s_ri[4][64];
+  __shared__ T s_k[4][64], s_rr[4][64], s_ri[4][64], s_rrl[4][64], s_ril[4][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t d = d0 + (int64_t)blockIdx.x * 4 + w;
   if (d >= d1) return;
@@ -293,16 +300,28 @@ __global__ __launch_bounds__(256) void k_rot_ll(
       s_k[w][lane] = (T)fk[i];
       s_rr[w][lane] = (T)rr;
       s_ri[w][lane] = (T)ri;
+      s_rrl[w][lane] = (T)(rr - (double)(T)rr);
+      s_ril[w][lane] = (T)(ri - (double)(T)ri);
     }
     __builtin_amdgcn_wave_barrier();
     const int n = (int)min((int64_t)64, o1 - base);
     for (int q = 0; q < n; ++q) {
       const T2* row = OT + (int64_t)s_j[w][q] * Hs + lane;
       const T kq = s_k[w][q], rr = s_rr[w][q], ri = s_ri[w][q];
+      const T rrl = s_rrl[w][q], ril = s_ril[w][q];
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const T2 o = row[64 * t];
-        const T zr = o.x + rr, zi = o.y + ri;
+        T zr, zi;
+        if constexpr (sizeof(T) == 4) {
+          // the hi parts cancel exactly when O ~ -R (Sterbenz); the lo parts restore the
+          // digits the FP32 roundings of O and R dropped
+          zr = (o.x + rr) + (o.z + rrl);
+          zi = (o.y + ri) + (o.w + ril);
+        } else {
+          zr = o.x + rr;
+          zi = o.y + ri;
+        }
         const T wv = sc * (zr * zr + zi * zi);
         acc[t] += (double)(kq * log_floor<T>(wv));
       }
@@ -317,7 +336,11 @@ __global__ __launch_bounds__(256) void k_rot_ll(
   for (int t = 0; t < NT; ++t) {
     const int h = lane + 64 * t;
     if (h < H) {
-      acc[t] -= scale * (normO[h] + nr + 2. * (double)X[(d - d0) * ldx + h]);
+      // cross term: the GEMM's K-slice partial products added in FP64, in slice order
+      const T* xp = X + (d - d0) * ldx + h;
+      double x = 0.;
+      for (int z = 0; z < nxs; ++z) x += (double)xp[z * xstride];
+      acc[t] -= scale * (normO[h] + nr + 2. * x);
       if (acc[t] > m) {           // first occurrence within the lane (h increases with t)
         m = acc[t];
         hbest = h;
@@ -775,6 +798,7 @@ struct rsp_rot {
   // forward work
   int mode = kSplit, mode_alloc = -1;
   int64_t chunk = 0, chunk_alloc = 0;
+  int xs = 1;                                   // K slices of the cross term (split form)
   void *d_A = nullptr, *d_OT = nullptr, *d_B = nullptr, *d_X = nullptr;
   double *d_normO = nullptr, *d_part = nullptr, *d_normR = nullptr;
   // merge
@@ -839,14 +863,15 @@ int ensure_work(rsp_rot* p, int64_t chunk) {
   const int64_t eb = elem_bytes(p->mode);
   const int64_t Bpad = xterm_pad_n(chunk);
   const int64_t ntiles = ceil_div(p->K, 64);
-  const int64_t ot_bytes = (int64_t)p->K * p->Hs * (p->mode == kFP64 ? 16 : 8);
+  const int64_t ot_bytes = (int64_t)p->K * p->Hs * 16;
+  const int xs = p->mode == kSplit ? xterm_split_k(p->Kp) : 1;
   int rc;
   char** A = reinterpret_cast<char**>(&p->d_A);
   char** B = reinterpret_cast<char**>(&p->d_B);
   char** OT = reinterpret_cast<char**>(&p->d_OT);
   char** X = reinterpret_cast<char**>(&p->d_X);
   if ((rc = dalloc(A, p->Mpad * p->Kp * eb)) || (rc = dalloc(B, Bpad * p->Kp * eb)) ||
-      (rc = dalloc(OT, ot_bytes)) || (rc = dalloc(X, chunk * p->Mpad * eb)) ||
+      (rc = dalloc(OT, ot_bytes)) || (rc = dalloc(X, xs * chunk * p->Mpad * eb)) ||
       (rc = dalloc(&p->d_normO, p->H)) || (rc = dalloc(&p->d_part, ntiles * p->H)) ||
       (rc = dalloc(&p->d_normR, chunk))) {
     free_work(p);
@@ -858,6 +883,7 @@ int ensure_work(rsp_rot* p, int64_t chunk) {
   RSP_HIP(hipMemset(p->d_OT, 0, (size_t)ot_bytes));
   p->mode_alloc = p->mode;
   p->chunk_alloc = chunk;
+  p->xs = xs;
   return kOk;
 }
 
@@ -937,8 +963,9 @@ void launch_ll_nt(rsp_rot* p, const Csr& f, const double* sx, const double* sy,
                   double* prob, double* logz, int32_t* best, hipStream_t st) {
   hipLaunchKernelGGL((k_rot_ll<T, NT>), dim3((unsigned)ceil_div(d1 - d0, 4)), dim3(256), 0, st,
                      p->size, p->H, p->Hs, p->rel, f.fo, f.fj, f.fk, p->d_scored, p->d_OT,
-                     p->d_normO, p->d_normR, p->d_X, p->Mpad, sx, sy, dia, scales, d0, d1, ll,
-                     prob, logz, best);
+                     p->d_normO, p->d_normR, p->d_X, p->Mpad, p->xs,
+                     p->chunk_alloc * p->Mpad, sx, sy, dia, scales, d0, d1, ll, prob, logz,
+                     best);
 }
 
 template <typename T>
@@ -995,7 +1022,7 @@ int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, cons
       const __bf16* ah = (const __bf16*)p->d_A;
       const __bf16* bh = (const __bf16*)p->d_B;
       rc = xterm_gemm_split(ah, ah + p->Mpad * p->Kp, bh, bh + Bpad * p->Kp, (float*)p->d_X,
-                            p->Mpad, n, p->Kp, p->Mpad, st);
+                            p->Mpad, n, p->Kp, p->Mpad, chunk * p->Mpad, st);
     }
     if (rc) return rc;
     if (p->timing) RSP_HIP(hipEventRecord(p->ev[4], st));

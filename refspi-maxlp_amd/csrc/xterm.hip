@@ -6,7 +6,8 @@
 //
 // Split-BF16: each FP32 operand x is carried as hi = bf16(x), lo = bf16(x - hi); the
 // product uses hi*hi + hi*lo + lo*hi (the lo*lo term, below 2^-16 relative, is dropped)
-// with FP32 accumulation in v_mfma_f32_32x32x16_bf16.
+// with FP32 accumulation in v_mfma_f32_32x32x16_bf16 over K slices of 6144 (split-K); the
+// slices' partial products are added in FP64 by their consumer.
 //
 // Kernel layout (k_gemm_split): 256 threads = 4 waves, macro tile 256 (r) x 128 (d),
 // each wave 128 x 64 = 4 x 2 MFMA tiles of 32 x 32.  K advances in stages of 32 bf16
@@ -39,6 +40,9 @@ constexpr int kTileB = kTN * kRowB;            // 8 KB per B plane tile
 constexpr int kLA = kTileA / 1024 / 4;         // DMA instructions per wave per A plane (4)
 constexpr int kLB = kTileB / 1024 / 4;         // per B plane (2)
 constexpr int kL = 2 * (kLA + kLB);            // loads per wave per stage (12)
+// K stages per split-K slice: 192 stages = 6144 K elements accumulated in FP32 per slice
+// (the slices are added in FP64 by the consumer).
+constexpr int kSliceStages = 192;
 static_assert(kL < 16, "vmcnt immediate");
 
 __shared__ __attribute__((aligned(16))) char s_ah0[kTileA], s_ah1[kTileA], s_ah2[kTileA];
@@ -129,21 +133,27 @@ struct SplitGemm {
   }
 };
 
+// K is cut into slices of kst stages (blockIdx.y = slice); slice z accumulates its stages
+// in FP32 and writes its partial product to C + z * slice_stride.
 __global__ __launch_bounds__(256, 1) void k_gemm_split(
     const __bf16* __restrict__ ah, const __bf16* __restrict__ al,
     const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, float* __restrict__ C,
-    int64_t N, int64_t Kp, int64_t ldc, int nM, int nN) {
+    int64_t N, int64_t Kp, int64_t ldc, int nM, int nN, int kst, int64_t slice_stride) {
   const int T = nM * nN;
   const int T8 = (T + 7) / 8;
   const int task = (int)(blockIdx.x & 7) * T8 + (int)(blockIdx.x >> 3);
   if (task >= T) return;
   const int mt = task % nM, nt = task / nM;
   const int64_t m0 = (int64_t)mt * kTM, n0 = (int64_t)nt * kTN;
+  const int z = blockIdx.y;
+  const int nkt = (int)(Kp / kBK);
+  const int64_t kz = (int64_t)z * kst * kBK;          // first K element of this slice
+  C += (int64_t)z * slice_stride;
   SplitGemm g;
-  g.ah = ah; g.al = al; g.bh = bh; g.bl = bl;
+  g.ah = ah + kz; g.al = al + kz; g.bh = bh + kz; g.bl = bl + kz;
   g.lane = threadIdx.x & 63;
   g.wave = threadIdx.x >> 6;
-  g.nk = (int)(Kp / kBK);
+  g.nk = min(kst, nkt - z * kst);
 #pragma unroll
   for (int u = 0; u < kLA; ++u) {
     const int s = (g.wave * kLA + u) * 64 + g.lane;
@@ -213,6 +223,17 @@ __global__ void k_split_rows(const float* __restrict__ X, int64_t rows, int64_t 
   *reinterpret_cast<bf16x8*>(lo + r * Kp + k0) = l;
 }
 
+// C[d * M + r] = sum over the S slices (FP64, fixed order) of P[z * stride + d * ldp + r].
+__global__ void k_slice_sum(const float* __restrict__ P, int64_t stride, int S, int64_t M,
+                            int64_t ldp, int64_t n, float* __restrict__ C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t d = i / M, r = i - d * M;
+  double s = 0.;
+  for (int z = 0; z < S; ++z) s += (double)P[z * stride + d * ldp + r];
+  C[i] = (float)s;
+}
+
 rocblas_handle g_blas = nullptr;
 int g_blas_dev = -1;
 
@@ -250,16 +271,21 @@ int xterm_split(const float* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi,
   return kOk;
 }
 
-// Split-BF16 GEMM on padded planes: C[d * ldc + r] for d < N, r < Mpad.
+int xterm_split_k(int64_t Kp) { return (int)ceil_div(Kp / kBK, kSliceStages); }
+
+// Split-BF16 GEMM on padded planes: partial products of the K slices at
+// C + z * slice_stride, [d * ldc + r] for d < N, r < Mpad, z < xterm_split_k(Kp).
 int xterm_gemm_split(const __bf16* ah, const __bf16* al, const __bf16* bh, const __bf16* bl,
                      float* C, int64_t Mpad, int64_t N, int64_t Kp, int64_t ldc,
-                     hipStream_t stream) {
+                     int64_t slice_stride, hipStream_t stream) {
   if (Mpad % kTM || Kp % kBK || ldc < Mpad || ldc % 4) return fail(kErrSize, "gemm padding");
+  const int S = xterm_split_k(Kp);
+  if (S > 1 && slice_stride < (N - 1) * ldc + Mpad) return fail(kErrSize, "gemm slice stride");
   if (N == 0 || Mpad == 0) return kOk;
   const int nM = (int)(Mpad / kTM), nN = (int)ceil_div(N, kTN);
   const int T8 = (nM * nN + 7) / 8;
-  hipLaunchKernelGGL(k_gemm_split, dim3(8 * T8), dim3(256), 0, stream, ah, al, bh, bl, C, N,
-                     Kp, ldc, nM, nN);
+  hipLaunchKernelGGL(k_gemm_split, dim3(8 * T8, S), dim3(256), 0, stream, ah, al, bh, bl, C, N,
+                     Kp, ldc, nM, nN, kSliceStages, slice_stride);
   RSP_LAUNCH_CHECK("k_gemm_split");
   return kOk;
 }
@@ -336,19 +362,23 @@ int rsp_xterm_gemm(int mode, int64_t M, int64_t N, int64_t K, const void* A, con
   float* Cp = nullptr;
   const size_t nplane = (size_t)(2 * Mp + 2 * Np) * Kp;
   RSP_HIP(hipMallocAsync((void**)&planes, nplane * sizeof(__bf16), stream));
-  hipError_t e = hipMallocAsync((void**)&Cp, (size_t)N * Mp * sizeof(float), stream);
+  hipError_t e = hipMallocAsync((void**)&Cp,
+                                (size_t)xterm_split_k(Kp) * N * Mp * sizeof(float), stream);
   if (e != hipSuccess) {
     (void)hipFreeAsync(planes, stream);
     return hip_fail(e, "hipMallocAsync");
   }
   __bf16 *ah = planes, *al = ah + Mp * Kp, *bh = al + Mp * Kp, *bl = bh + Np * Kp;
+  const int S = xterm_split_k(Kp);
   int rc = xterm_split((const float*)A, M, K, K, ah, al, Mp, Kp, stream);
   if (!rc) rc = xterm_split((const float*)B, N, K, K, bh, bl, Np, Kp, stream);
-  if (!rc) rc = xterm_gemm_split(ah, al, bh, bl, Cp, Mp, N, Kp, Mp, stream);
+  if (!rc) rc = xterm_gemm_split(ah, al, bh, bl, Cp, Mp, N, Kp, Mp, N * Mp, stream);
   if (!rc) {
-    hipError_t ce = hipMemcpy2DAsync(C, M * sizeof(float), Cp, Mp * sizeof(float),
-                                     M * sizeof(float), N, hipMemcpyDeviceToDevice, stream);
-    if (ce != hipSuccess) rc = hip_fail(ce, "hipMemcpy2DAsync");
+    const int64_t n = N * M;
+    hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, Cp,
+                       N * Mp, S, M, Mp, n, (float*)C);
+    hipError_t ce = hipGetLastError();
+    if (ce != hipSuccess) rc = hip_fail(ce, "k_slice_sum");
   }
   (void)hipFreeAsync(Cp, stream);
   (void)hipFreeAsync(planes, stream);

@@ -16,10 +16,16 @@ int64_t xterm_pad_k(int64_t K);
 int xterm_split(const float* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi, __bf16* lo,
                 int64_t rows_pad, int64_t Kp, hipStream_t stream);
 
-// C[d * ldc + r] = sum_k A[r, k] B[d, k] on padded bf16 hi/lo planes (split products).
+// Number of K slices of the split GEMM for padded K = Kp (each slice accumulates 6144 K
+// elements in FP32).
+int xterm_split_k(int64_t Kp);
+
+// Partial products sum_{k in slice z} A[r, k] B[d, k] on padded bf16 hi/lo planes (split
+// products) at C + z * slice_stride + d * ldc + r, for z < xterm_split_k(Kp); the caller
+// adds the slices (in FP64).
 int xterm_gemm_split(const __bf16* ah, const __bf16* al, const __bf16* bh, const __bf16* bl,
                      float* C, int64_t Mpad, int64_t N, int64_t Kp, int64_t ldc,
-                     hipStream_t stream);
+                     int64_t slice_stride, hipStream_t stream);
 
 // The same through rocBLAS in FP32 / FP64 (row-major A [M x lda], B [N x ldb]).
 int xterm_gemm_blas32(const float* A, const float* B, float* C, int64_t M, int64_t N,
