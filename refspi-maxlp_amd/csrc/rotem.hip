@@ -21,9 +21,10 @@
 //                  pixel-major copy OT[j][h] through an LDS transpose, per-tile |O_h|^2.
 //   k_ref_rows     one work-group per frame: S over the distinct radii in LDS, separable
 //                  phase ramps, GEMM operand rows (B) and |R_d|^2.
-//   k_rot_ll       one wave per frame, lanes over hypotheses: sparse log term over the
-//                  frame's photons (OT rows are coalesced 64-hypothesis reads), the dense
-//                  term, the FP64 log-sum-exp and the first-occurrence argmax.
+//   k_rot_lt       one wave per (frame, 64-hypothesis tile), tiles outermost: sparse log
+//                  term over the frame's photons (OT row segments are coalesced 64-hypothesis
+//                  reads of a cache-resident tile) and the dense term -> LL.
+//   k_rot_lse      one wave per frame: FP64 log-sum-exp over LL, first-occurrence argmax.
 //   k_merge_view   one work-group per pixel (pixel-major photon lists): VT[j][h] +=
 //                  sum_d K P(h|d), no atomics.
 //   k_merge_model  one thread per (model pixel, class): slice_merge in gather form (the
@@ -52,6 +53,7 @@ using namespace rsp;
 namespace {
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 constexpr double kPi = 3.14159265358979323846;
 
 enum Mode { kFP32 = 0, kSplit = 1, kFP64 = 2 };
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(256) void k_rot_slices(
     if constexpr (MODE == kFP64) {
       ((double2*)OT)[(int64_t)jj * Hs + h] = v;
     } else {
-      // FP32 hi + lo: O = hi + lo to about 2^-48 relative (see k_rot_ll)
+      // FP32 hi + lo: O = hi + lo to about 2^-48 relative (see k_rot_lt)
       const float hr = (float)v.x, hi = (float)v.y;
       ((float4*)OT)[(int64_t)jj * Hs + h] =
           make_float4(hr, hi, (float)(v.x - (double)hr), (float)(v.y - (double)hi));
@@ -197,29 +199,52 @@ __global__ __launch_bounds__(256) void k_ref_rows(
     ey[i] = make_double2(c, s);
   }
   __syncthreads();
+  // four consecutive pixels per thread (one 16 B bf16 store per plane, or 32 B FP32); the
+  // row padding j in [K, Kp/2) is written as zeros
   double acc = 0.;
-  for (int j = threadIdx.x; j < K; j += 256) {
-    const int t = scored[j];
-    const int x = t / size, y = t - (t / size) * size;
-    const double2 a = ex[x], b = ey[y];
-    const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
-    const double sv = S[jrad[j]];
-    const double vr = sv * pr, vi = sv * pi;
-    acc += vr * vr + vi * vi;
-    const int64_t k = dl * Kp + 2 * j;
+  const int jpad = (int)(Kp / 2);
+  for (int j0 = 4 * threadIdx.x; j0 < jpad; j0 += 4 * 256) {
+    double vr[4], vi[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + u;
+      vr[u] = 0.;
+      vi[u] = 0.;
+      if (j < K) {
+        const int t = scored[j];
+        const int x = t / size, y = t - (t / size) * size;
+        const double2 a = ex[x], b = ey[y];
+        const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
+        const double sv = S[jrad[j]];
+        vr[u] = sv * pr;
+        vi[u] = sv * pi;
+        acc += vr[u] * vr[u] + vi[u] * vi[u];
+      }
+    }
+    const int64_t k = dl * Kp + 2 * j0;
     if constexpr (MODE == kFP32) {
-      *reinterpret_cast<float2*>((float*)B + k) = make_float2((float)vr, (float)vi);
+      float4* o = reinterpret_cast<float4*>((float*)B + k);
+      o[0] = make_float4((float)vr[0], (float)vi[0], (float)vr[1], (float)vi[1]);
+      o[1] = make_float4((float)vr[2], (float)vi[2], (float)vr[3], (float)vi[3]);
     } else if constexpr (MODE == kFP64) {
-      *reinterpret_cast<double2*>((double*)B + k) = make_double2(vr, vi);
+      double2* o = reinterpret_cast<double2*>((double*)B + k);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) o[u] = make_double2(vr[u], vi[u]);
     } else {
       __bf16* bh = (__bf16*)B;
       __bf16* bl = bh + Bpad * Kp;
-      const float fr = (float)vr, fi = (float)vi;
-      const __bf16 hr = (__bf16)fr, hi = (__bf16)fi;
-      bf16x2 hv = {hr, hi};
-      bf16x2 lv = {(__bf16)(fr - (float)hr), (__bf16)(fi - (float)hi)};
-      *reinterpret_cast<bf16x2*>(bh + k) = hv;
-      *reinterpret_cast<bf16x2*>(bl + k) = lv;
+      bf16x8 hv, lv;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float fr = (float)vr[u], fi = (float)vi[u];
+        const __bf16 hr = (__bf16)fr, hi = (__bf16)fi;
+        hv[2 * u] = hr;
+        hv[2 * u + 1] = hi;
+        lv[2 * u] = (__bf16)(fr - (float)hr);
+        lv[2 * u + 1] = (__bf16)(fi - (float)hi);
+      }
+      *reinterpret_cast<bf16x8*>(bh + k) = hv;
+      *reinterpret_cast<bf16x8*>(bl + k) = lv;
     }
   }
   red[threadIdx.x] = acc;
@@ -262,10 +287,14 @@ __device__ __forceinline__ T log_floor(T w) {
     return w >= 1.e-20 ? log(w) : -1000.;
 }
 
-// One wave per frame of [d0, d1): LL, P, log Z, argmax.  Hypotheses h = lane + 64 t.  The
-// cross term X[d - d0, h] is the sum of nxs slices xstride elements apart.
-template <typename T, int NT>
-__global__ __launch_bounds__(256) void k_rot_ll(
+// LL of hypothesis tile blockIdx.y (h = 64 * tile + lane) for the frames [d0, d1), one
+// wave per frame: the sparse log term over the frame's photons (OT row segments of the tile,
+// coalesced) and the dense term -scale (|O_h|^2 + |R_d|^2 + 2 X), X being the sum of nxs
+// K-slice partial products xstride elements apart (added in FP64).  Tile-major launch order
+// (blockIdx.y outermost) keeps one tile's OT columns (K x 64 entries) cache-resident while
+// every frame visits them.  Output llb[(d - d0) * H + h] (FP64).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rot_lt(
     int size, int H, int Hs, double rel, const int64_t* __restrict__ fo,
     const int32_t* __restrict__ fj, const float* __restrict__ fk,
     const int32_t* __restrict__ scored, const void* __restrict__ OT_,
@@ -273,8 +302,7 @@ __global__ __launch_bounds__(256) void k_rot_ll(
     const void* __restrict__ X_, int64_t ldx, int nxs, int64_t xstride,
     const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ dia, const double* __restrict__ scales, int64_t d0, int64_t d1,
-    double* __restrict__ ll, double* __restrict__ prob, double* __restrict__ logz,
-    int32_t* __restrict__ best) {
+    double* __restrict__ llb) {
   // OT entries: FP64 (re, im), or FP32 (re_hi, im_hi, re_lo, im_lo)
   typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type T2;
   __shared__ int32_t s_j[4][64];
@@ -282,13 +310,12 @@ __global__ __launch_bounds__(256) void k_rot_ll(
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t d = d0 + (int64_t)blockIdx.x * 4 + w;
   if (d >= d1) return;
-  const T2* OT = (const T2*)OT_;
+  const int h = (int)blockIdx.y * 64 + lane;
+  const T2* OT = (const T2*)OT_ + h;
   const double scale = scales[d];
   const T sc = (T)scale;
   const double fsx = sx[d], fsy = sy[d], fd = dia[d];
-  double acc[NT];
-#pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = 0.;
+  double acc = 0.;
   const int64_t o0 = fo[d], o1 = fo[d + 1];
   for (int64_t base = o0; base < o1; base += 64) {
     const int64_t i = base + lane;
@@ -306,45 +333,48 @@ __global__ __launch_bounds__(256) void k_rot_ll(
     __builtin_amdgcn_wave_barrier();
     const int n = (int)min((int64_t)64, o1 - base);
     for (int q = 0; q < n; ++q) {
-      const T2* row = OT + (int64_t)s_j[w][q] * Hs + lane;
+      const T2 o = OT[(int64_t)s_j[w][q] * Hs];
       const T kq = s_k[w][q], rr = s_rr[w][q], ri = s_ri[w][q];
-      const T rrl = s_rrl[w][q], ril = s_ril[w][q];
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const T2 o = row[64 * t];
-        T zr, zi;
-        if constexpr (sizeof(T) == 4) {
-          // the hi parts cancel exactly when O ~ -R (Sterbenz); the lo parts restore the
-          // digits the FP32 roundings of O and R dropped
-          zr = (o.x + rr) + (o.z + rrl);
-          zi = (o.y + ri) + (o.w + ril);
-        } else {
-          zr = o.x + rr;
-          zi = o.y + ri;
-        }
-        const T wv = sc * (zr * zr + zi * zi);
-        acc[t] += (double)(kq * log_floor<T>(wv));
+      T zr, zi;
+      if constexpr (sizeof(T) == 4) {
+        // the hi parts cancel exactly when O ~ -R (Sterbenz); the lo parts restore the
+        // digits the FP32 roundings of O and R dropped
+        zr = (o.x + rr) + (o.z + s_rrl[w][q]);
+        zi = (o.y + ri) + (o.w + s_ril[w][q]);
+      } else {
+        zr = o.x + rr;
+        zi = o.y + ri;
       }
+      const T wv = sc * (zr * zr + zi * zi);
+      acc += (double)(kq * log_floor<T>(wv));
     }
     __builtin_amdgcn_wave_barrier();
   }
-  const double nr = normR[d - d0];
-  const T* X = (const T*)X_;
+  if (h >= H) return;
+  const T* xp = (const T*)X_ + (d - d0) * ldx + h;
+  double x = 0.;
+  for (int z = 0; z < nxs; ++z) x += (double)xp[z * xstride];
+  llb[(d - d0) * H + h] = acc - scale * (normO[h] + normR[d - d0] + 2. * x);
+}
+
+// One wave per frame of [d0, d1): the FP64 log-sum-exp over the frame's LL row
+// (llb[(d - d0) * H + h]) -> P, log Z and the first-occurrence argmax.
+__global__ __launch_bounds__(256) void k_rot_lse(int H, int64_t d0, int64_t d1,
+                                                 const double* __restrict__ llb,
+                                                 double* __restrict__ prob,
+                                                 double* __restrict__ logz,
+                                                 int32_t* __restrict__ best) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t d = d0 + (int64_t)blockIdx.x * 4 + w;
+  if (d >= d1) return;
+  const double* row = llb + (d - d0) * H;
   double m = -INFINITY;
   int hbest = 0x7fffffff;
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int h = lane + 64 * t;
-    if (h < H) {
-      // cross term: the GEMM's K-slice partial products added in FP64, in slice order
-      const T* xp = X + (d - d0) * ldx + h;
-      double x = 0.;
-      for (int z = 0; z < nxs; ++z) x += (double)xp[z * xstride];
-      acc[t] -= scale * (normO[h] + nr + 2. * x);
-      if (acc[t] > m) {           // first occurrence within the lane (h increases with t)
-        m = acc[t];
-        hbest = h;
-      }
+  for (int h = lane; h < H; h += 64) {
+    const double v = row[h];
+    if (v > m) {                  // first occurrence within the lane (h increases)
+      m = v;
+      hbest = h;
     }
   }
   const double lm = m;
@@ -353,17 +383,10 @@ __global__ __launch_bounds__(256) void k_rot_ll(
   int hb = (lm == m) ? hbest : 0x7fffffff;
   for (int o = 32; o > 0; o >>= 1) hb = min(hb, __shfl_xor(hb, o));
   double s = 0.;
-#pragma unroll
-  for (int t = 0; t < NT; ++t)
-    if (lane + 64 * t < H) s += exp(acc[t] - m);
+  for (int h = lane; h < H; h += 64) s += exp(row[h] - m);
   s = wave_sum(s);
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int h = lane + 64 * t;
-    if (h >= H) continue;
-    if (ll) ll[d * H + h] = acc[t];
-    if (prob) prob[d * H + h] = exp(acc[t] - m) / s;
-  }
+  if (prob)
+    for (int h = lane; h < H; h += 64) prob[d * H + h] = exp(row[h] - m) / s;
   if (lane == 0) {
     if (logz) logz[d] = m + log(s);
     if (best) best[d] = hb;
@@ -800,6 +823,7 @@ struct rsp_rot {
   int64_t chunk = 0, chunk_alloc = 0;
   int xs = 1;                                   // K slices of the cross term (split form)
   void *d_A = nullptr, *d_OT = nullptr, *d_B = nullptr, *d_X = nullptr;
+  double* d_LLc = nullptr;                      // LL of a GEMM chunk (when not asked for)
   double *d_normO = nullptr, *d_part = nullptr, *d_normR = nullptr;
   // merge
   double *d_VT = nullptr, *d_wsum = nullptr, *d_wpart = nullptr;
@@ -827,6 +851,7 @@ void free_work(rsp_rot* p) {
   dfree(p->d_OT);
   dfree(p->d_B);
   dfree(p->d_X);
+  dfree(p->d_LLc);
   dfree(p->d_normO);
   dfree(p->d_part);
   dfree(p->d_normR);
@@ -872,6 +897,7 @@ int ensure_work(rsp_rot* p, int64_t chunk) {
   char** X = reinterpret_cast<char**>(&p->d_X);
   if ((rc = dalloc(A, p->Mpad * p->Kp * eb)) || (rc = dalloc(B, Bpad * p->Kp * eb)) ||
       (rc = dalloc(OT, ot_bytes)) || (rc = dalloc(X, xs * chunk * p->Mpad * eb)) ||
+      (rc = dalloc(&p->d_LLc, chunk * p->H)) ||
       (rc = dalloc(&p->d_normO, p->H)) || (rc = dalloc(&p->d_part, ntiles * p->H)) ||
       (rc = dalloc(&p->d_normR, chunk))) {
     free_work(p);
@@ -957,32 +983,24 @@ int launch_ref_to(rsp_rot* p, const double* sx, const double* sy, const double* 
   return kOk;
 }
 
-template <typename T, int NT>
-void launch_ll_nt(rsp_rot* p, const Csr& f, const double* sx, const double* sy,
-                  const double* dia, const double* scales, int64_t d0, int64_t d1, double* ll,
-                  double* prob, double* logz, int32_t* best, hipStream_t st) {
-  hipLaunchKernelGGL((k_rot_ll<T, NT>), dim3((unsigned)ceil_div(d1 - d0, 4)), dim3(256), 0, st,
-                     p->size, p->H, p->Hs, p->rel, f.fo, f.fj, f.fk, p->d_scored, p->d_OT,
-                     p->d_normO, p->d_normR, p->d_X, p->Mpad, p->xs,
-                     p->chunk_alloc * p->Mpad, sx, sy, dia, scales, d0, d1, ll, prob, logz,
-                     best);
-}
-
+// LL of frames [d0, d1) into ll (rows d of the CSR's frames) or the chunk buffer, then the
+// log-sum-exp.
 template <typename T>
 int launch_ll(rsp_rot* p, const Csr& f, const double* sx, const double* sy, const double* dia,
               const double* scales, int64_t d0, int64_t d1, double* ll, double* prob,
               double* logz, int32_t* best, hipStream_t st) {
-  switch (p->NT) {
-#define RSP_NT(n)                                                                          \
-  case n:                                                                                  \
-    launch_ll_nt<T, n>(p, f, sx, sy, dia, scales, d0, d1, ll, prob, logz, best, st);      \
-    break;
-    RSP_NT(1) RSP_NT(2) RSP_NT(3) RSP_NT(4) RSP_NT(5) RSP_NT(6) RSP_NT(8) RSP_NT(10)
-    RSP_NT(12) RSP_NT(16) RSP_NT(20) RSP_NT(24)
-#undef RSP_NT
-    default: return fail(kErrUnsupported, "hypothesis count not supported");
+  double* llb = ll ? ll + d0 * p->H : p->d_LLc;
+  const unsigned fb = (unsigned)ceil_div(d1 - d0, 4);
+  hipLaunchKernelGGL((k_rot_lt<T>), dim3(fb, (unsigned)ceil_div(p->H, 64)), dim3(256), 0, st,
+                     p->size, p->H, p->Hs, p->rel, f.fo, f.fj, f.fk, p->d_scored, p->d_OT,
+                     p->d_normO, p->d_normR, p->d_X, p->Mpad, p->xs,
+                     p->chunk_alloc * p->Mpad, sx, sy, dia, scales, d0, d1, llb);
+  RSP_LAUNCH_CHECK("k_rot_lt");
+  if (prob || logz || best) {
+    hipLaunchKernelGGL(k_rot_lse, dim3(fb), dim3(256), 0, st, p->H, d0, d1, llb, prob, logz,
+                       best);
+    RSP_LAUNCH_CHECK("k_rot_lse");
   }
-  RSP_LAUNCH_CHECK("k_rot_ll");
   return kOk;
 }
 

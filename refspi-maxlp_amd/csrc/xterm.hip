@@ -6,7 +6,7 @@
 //
 // Split-BF16: each FP32 operand x is carried as hi = bf16(x), lo = bf16(x - hi); the
 // product uses hi*hi + hi*lo + lo*hi (the lo*lo term, below 2^-16 relative, is dropped)
-// with FP32 accumulation in v_mfma_f32_32x32x16_bf16 over K slices of 6144 (split-K); the
+// with FP32 accumulation in v_mfma_f32_32x32x16_bf16 over K slices of 12288 (split-K); the
 // slices' partial products are added in FP64 by their consumer.
 //
 // Kernel layout (k_gemm_split): 256 threads = 4 waves, macro tile 256 (r) x 128 (d),
@@ -40,9 +40,9 @@ constexpr int kTileB = kTN * kRowB;            // 8 KB per B plane tile
 constexpr int kLA = kTileA / 1024 / 4;         // DMA instructions per wave per A plane (4)
 constexpr int kLB = kTileB / 1024 / 4;         // per B plane (2)
 constexpr int kL = 2 * (kLA + kLB);            // loads per wave per stage (12)
-// K stages per split-K slice: 192 stages = 6144 K elements accumulated in FP32 per slice
+// K stages per split-K slice: 384 stages = 12288 K elements accumulated in FP32 per slice
 // (the slices are added in FP64 by the consumer).
-constexpr int kSliceStages = 192;
+constexpr int kSliceStages = 384;
 static_assert(kL < 16, "vmcnt immediate");
 
 __shared__ __attribute__((aligned(16))) char s_ah0[kTileA], s_ah1[kTileA], s_ah2[kTileA];

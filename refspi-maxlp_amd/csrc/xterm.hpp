@@ -16,7 +16,7 @@ int64_t xterm_pad_k(int64_t K);
 int xterm_split(const float* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi, __bf16* lo,
                 int64_t rows_pad, int64_t Kp, hipStream_t stream);
 
-// Number of K slices of the split GEMM for padded K = Kp (each slice accumulates 6144 K
+// Number of K slices of the split GEMM for padded K = Kp (each slice accumulates 12288 K
 // elements in FP32).
 int xterm_split_k(int64_t Kp);
 
