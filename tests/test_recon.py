@@ -124,7 +124,9 @@ def test_dropin_module_names():
         assert importlib.import_module('emc').EMC is emc.EMC
         assert importlib.import_module('maxLP').MaxLPhaser is maxlp.MaxLPhaser
         assert importlib.import_module('dset').Dataset is dset.Dataset
+        from refspi_amd import frc
+        assert importlib.import_module('calc_frc').FRC is frc.FRC
     finally:
         sys.path.remove(d)
-        for m in ('emc', 'maxLP', 'dset'):
+        for m in ('emc', 'maxLP', 'dset', 'calc_frc'):
             sys.modules.pop(m, None)
