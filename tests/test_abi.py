@@ -40,3 +40,29 @@ def test_argument_errors_without_gpu():
     assert lib.rsp_emc_estep(None, None, None, None, None, None, None, None) == -1
     h = C.c_void_p()
     assert lib.rsp_emc_create(0, 4, 1, None, None, None, None, None, 1, C.byref(h)) == -1
+
+
+def test_rotation_form_argument_errors_without_gpu():
+    """The rotation-form entry points reject null plans / arguments before any device call."""
+    import ctypes as C
+    from refspi_amd import _lib
+    lib = _lib.load()
+    assert lib.rsp_rot_set_option(None, 1, 0) == -1
+    assert lib.rsp_rot_estep(None, None, None, None, None, None, None, None, None, None) == -1
+    assert lib.rsp_rot_estep_best(None, *([None] * 10)) == -1
+    assert lib.rsp_rot_merge(None, None, None, None, None, None, None) == -1
+    assert lib.rsp_rot_adjoint(None, *([None] * 11)) == -1
+    n = C.c_int64()
+    assert lib.rsp_rot_pack(None, 1, None, None, None, None, None, None, None, None, None,
+                            None, None, C.byref(n)) == -1
+    assert lib.rsp_rot_pass_begin(None, None, 0, 1, None) == -1
+    assert lib.rsp_rot_pass_chunk(None, 1, *([None] * 18)) == -1
+    assert lib.rsp_rot_pass_end(None, None, None, None, None, None) == -1
+    assert lib.rsp_rot_create(0, 64, 8, None, None, 1., None) == -1
+    assert b'null' in lib.rsp_last_error()
+    out = C.c_void_p()
+    cs = (C.c_double * 8)()
+    assert lib.rsp_rot_create(0, 4, 8, cs, cs, 1., C.byref(out)) == -2       # size < 9
+    assert lib.rsp_rot_create(0, 64, 0, cs, cs, 1., C.byref(out)) == -2      # no rotations
+    assert lib.rsp_rot_create(0, 64, 2000, cs, cs, 1., C.byref(out)) == -2   # > 1536
+    assert lib.rsp_xterm_gemm(1, -1, 1, 1, None, None, None, None) == -2
