@@ -319,7 +319,7 @@ def main():
                     help='LDS bank model of the photon scheduler (0 = none, 32, 64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-frames', type=int, default=1000)
-    ap.add_argument('--cpu-states', type=int, default=10)
+    ap.add_argument('--cpu-states', type=int, default=30)
     ap.add_argument('--stream-frames', type=int, default=500_000)
     ap.add_argument('--stream-chunk', type=int, default=50_000)
     ap.add_argument('--refine-steps', type=int, default=3)
