@@ -36,14 +36,16 @@ namespace rsp {
 constexpr int kNPMax = 4;        // table parts (LDS-sized pieces of the scored annulus)
 // The scored annulus is cut into parts of kPartMax entries; a part buffer in LDS holds
 // kChunks 1 KB chunks (kPStride floats: the entries, then >= 32 zero pad entries), and a
-// workgroup keeps kNBuf part buffers (100 KB of the 160 KB LDS) fed by asynchronous
-// direct-to-LDS loads one step ahead of the gather.  Both buffers lie within the 16-bit
-// immediate offset of ds_read (at 0 and 50176 B), so the slot registers address either
-// buffer without per-slot address arithmetic.
+// workgroup keeps kNBuf part buffers (147 KB of the 160 KB LDS) fed by asynchronous
+// direct-to-LDS loads two steps ahead of the gather.  The first two buffers lie within
+// the 16-bit immediate offset of ds_read (at 0 and 50176 B), so the slot registers
+// address them without per-slot address arithmetic; reads of the third (at 100352 B) add
+// kRebase to the slot register and take the rest of its base as the immediate offset.
 constexpr int kChunks = 49;
 constexpr int kPStride = kChunks * 256;
 constexpr int kPartMax = kPStride - 32;
-constexpr int kNBuf = 2;
+constexpr int kNBuf = 3;
+constexpr uint32_t kRebase = 2 * kPStride * 4 - 64512;  // third buffer's base - offset
 
 struct Top3 {
   double v1, v2, v3;
@@ -184,36 +186,49 @@ __global__ __launch_bounds__(256) void k_tables(
 // overlap the arithmetic of the previous one.  Each block is summed pairwise in FP32
 // (error <= 3 * 2^-24 * sum|v|, accounted for in the per-frame bound of k_resolve) and
 // added to an FP64 accumulator.
-template <int K, int P>
-__device__ __forceinline__ void load8(const char* lds, const uint32_t (&a)[P], float (&v)[8]) {
+// rb == kRebase for reads of the third buffer (one add per read: its base exceeds the
+// 16-bit immediate offset), else 0 (unused)
+template <int K, int P, bool REB>
+__device__ __forceinline__ void load8(const char* lds, const uint32_t (&a)[P], float (&v)[8],
+                                      uint32_t rb) {
+  if constexpr (REB) {
+    const char* base = lds - kRebase;  // within the immediate offset
 #pragma unroll
-  for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K + u]);
+    for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(base + (a[K + u] + rb));
+  } else {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K + u]);
+  }
 }
 
 // Software pipeline: the LDS reads of block K+8 are issued before the sums of block K, so
 // every wave keeps 8 reads in flight while it adds instead of waiting for all of a part's
 // reads at once.  The empty asm statements pin the order: each block's sum is complete
 // before the next block's reads are issued (memory clobber), and cannot sink below them.
-template <int K, int K1, int P, int B>
+template <int K, int K1, int P, int B, bool REB>
 __device__ __forceinline__ void gather_pipe(const char* lds, const uint32_t (&a)[P],
-                                            float (&v)[2][8], double (&acc)[4]) {
+                                            float (&v)[2][8], double (&acc)[4], uint32_t rb) {
   if constexpr (K < K1) {
-    if constexpr (K + 8 < K1) load8<K + 8, P>(lds, a, v[B ^ 1]);
+    if constexpr (K + 8 < K1) load8<K + 8, P, REB>(lds, a, v[B ^ 1], rb);
     const float* w = v[B];
     float s8 = ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
     asm volatile("" : "+v"(s8) : : "memory");
     acc[(K >> 3) & 3] += (double)s8;
-    gather_pipe<K + 8, K1, P, B ^ 1>(lds, a, v, acc);
+    gather_pipe<K + 8, K1, P, B ^ 1, REB>(lds, a, v, acc, rb);
   }
 }
 
-template <int K0, int K1, int P>
+template <int K0, int K1, int P, bool REB>
 __device__ __forceinline__ void gather_range(const char* lds, const uint32_t (&a)[P],
                                              double (&acc)[4]) {
   if constexpr (K0 < K1) {
+    // an opaque copy of kRebase made at every step: the adds that use it cannot be hoisted
+    // out of the hypothesis loop (which would hold one address register per slot)
+    uint32_t rb = 0;
+    if constexpr (REB) asm volatile("" : "=s"(rb) : "0"(kRebase));
     float v[2][8];
-    load8<K0, P>(lds, a, v[0]);
-    gather_pipe<K0, K1, P, 0>(lds, a, v, acc);
+    load8<K0, P, REB>(lds, a, v[0], rb);
+    gather_pipe<K0, K1, P, 0, REB>(lds, a, v, acc, rb);
   }
 }
 
@@ -233,18 +248,19 @@ __device__ __forceinline__ void load_slots(uint32_t (&a)[P], const uint32_t* __r
 // --------------------------------------------------------------------------------
 // The two LDS part buffers of the gather, as separate objects: the compiler then knows
 // which buffer each direct-to-LDS load writes and waits only for the buffer it reads.
-__shared__ float g_buf0[kPStride], g_buf1[kPStride];
+__shared__ float g_buf0[kPStride], g_buf1[kPStride], g_buf2[kPStride];
 
 template <int B>
 __device__ __forceinline__ float* part_buf() {
   if constexpr (B == 0) return g_buf0;
-  else return g_buf1;
+  else if constexpr (B == 1) return g_buf1;
+  else return g_buf2;
 }
 
-// The gather walks steps q = (hypothesis, part) in groups of 2 hypotheses so that the
-// part index and the LDS buffer (q % 2) of every step are compile-time constants.  Step
+// The gather walks steps q = (hypothesis, part) in groups of 3 hypotheses so that the
+// part index and the LDS buffer (q % 3) of every step are compile-time constants.  Step
 // q: wait until its part is in LDS (this wave's loads via vmcnt, every wave's via the
-// barrier), start the loads of step q+1 into the buffer freed by step q-1, then gather the
+// barrier), start the loads of step q+2 into the buffer freed by step q-1, then gather the
 // part's S slots.  Template: S slots per part and lane, NP parts, NW waves, L loads/wave.
 template <int S, int NP, int NW, int L, int X>
 struct GatherStep {
@@ -275,26 +291,27 @@ struct GatherStep {
                                        const double* __restrict__ init, double (&acc)[4],
                                        Top3& top) const {
     constexpr int PP = QS % NP;
-    const int q = g * 2 * NP + QS;
-    const int h = h0 + g * 2 + QS / NP;
+    const int q = g * 3 * NP + QS;
+    const int h = h0 + g * 3 + QS / NP;
     // s_waitcnt through the builtin so that the compiler's own wait bookkeeping sees it
-    // (gfx9 encoding: vmcnt[3:0], expcnt[6:4] = 7, lgkmcnt[11:8] = 15: no wait on those)
+    // (gfx9 encoding: vmcnt[3:0], expcnt[6:4] = 7 (no wait), lgkmcnt[11:8])
     static_assert(L < 16, "vmcnt immediate");
-    // vmcnt = 0: this wave's loads of step q (the only ones in flight) have landed;
+    // vmcnt = L: this wave's loads of step q have landed (the L loads of step q+1, issued
+    // after them, may still be in flight: vector memory completes in order);
     // lgkmcnt = 0: this wave's LDS reads of step q-1 have completed -- the loads issued
     // after the barrier overwrite the buffer step q-1 read, so no read of it may still be
     // in flight in any wave
-    __builtin_amdgcn_s_waitcnt(0x070);
+    __builtin_amdgcn_s_waitcnt(0x070 | L);
     // a bare barrier: a fenced __syncthreads would also wait for the loads of step q+1
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    issue(q + 1, part_buf<(QS + 1) % 2>());
+    issue(q + 2, part_buf<(QS + 2) % 3>());
     if constexpr (PP == 0) {
       acc[0] = init[min(h, h1 - 1) / R] * scale;
       acc[1] = acc[2] = acc[3] = 0.;
     }
-    gather_range<PP * S, PP * S + S, P>(reinterpret_cast<const char*>(part_buf<QS % 2>()), a,
-                                        acc);
+    gather_range<PP * S, PP * S + S, P, QS % 3 == 2>(
+        reinterpret_cast<const char*>(part_buf<QS % 3>()), a, acc);
     if constexpr (PP == NP - 1) {
       double tot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
       // a frame split over X lanes: sum the X partial sums (only its first lane carries
@@ -309,7 +326,7 @@ struct GatherStep {
   __device__ __forceinline__ void group(int g, const uint32_t (&a)[P],
                                         const double* __restrict__ init, double (&acc)[4],
                                         Top3& top) const {
-    if constexpr (QS < 2 * NP) {
+    if constexpr (QS < 3 * NP) {
       step<QS>(g, a, init, acc, top);
       group<QS + 1>(g, a, init, acc, top);
     }
@@ -344,12 +361,13 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   load_slots<0, P>(a, slots + (int64_t)lw * P * 64 + lane);
   Top3 top = top3_empty();
   // every wave issues exactly L 1 KB direct-to-LDS loads per step (surplus loads repeat the
-  // last chunk with identical bytes); one step is in flight at a time
+  // last chunk with identical bytes); two steps are in flight ahead of the gather
   const GatherStep<S, NP, NW, L, X> st{tables, TS, hb, h0, h1, (h1 - h0) * NP, R, wid, lane,
                                     scale_sorted[j]};
   st.issue(0, g_buf0);
+  st.issue(1, g_buf1);
   double acc[4];
-  const int ngroups = (h1 - h0 + 1) / 2;
+  const int ngroups = (h1 - h0 + 2) / 3;
   for (int g = 0; g < ngroups; ++g) st.template group<0>(g, a, init, acc, top);
   // no direct-to-LDS write may still be in flight when the workgroup's LDS is released
   __builtin_amdgcn_s_waitcnt(0xF70);
