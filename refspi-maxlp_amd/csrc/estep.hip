@@ -350,7 +350,9 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
     seg = b / n_tiles;
     tile = b % n_tiles;
   }
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index as a scalar: the table loads' LDS destinations and chunk offsets are then
+  // scalar too (no address registers per load)
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int lw = tile * NW + wid;  // wave within this slot class
   const int gw = gw0 + lw;         // global wave
   const int j = gw * 64 + lane;
@@ -392,11 +394,21 @@ __global__ __launch_bounds__(256) void k_merge_top3(const Top3* __restrict__ par
   for (int c = sm.n - 1; c >= 0; --c)
     if (gw < sm.gw_end[c]) nseg = sm.nseg[c];
   Top3 t = first ? top3_empty() : running[j];
-  for (int g = 0; g < nseg; ++g) {
-    const Top3 p = partials[(int64_t)g * Npad + j];
-    top3_insert(t, p.v1, p.h1);
-    top3_insert(t, p.v2, p.h2);
-    if (p.v3 > t.v3) t.v3 = p.v3;
+  // the partials of 8 segments are loaded before any is merged: up to ~200 segments per
+  // frame would otherwise cost one memory latency each
+  constexpr int kU = 8;
+  for (int g0 = 0; g0 < nseg; g0 += kU) {
+    Top3 p[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (g0 + u < nseg) p[u] = partials[(int64_t)(g0 + u) * Npad + j];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (g0 + u < nseg) {
+        top3_insert(t, p[u].v1, p[u].h1);
+        top3_insert(t, p[u].v2, p[u].h2);
+        if (p[u].v3 > t.v3) t.v3 = p[u].v3;
+      }
   }
   running[j] = t;
 }
@@ -691,13 +703,19 @@ using GatherFn = void (*)(const float*, int, int, int, int, int, int, const uint
                           const double*, const double*, int, Top3*, int);
 
 // Gather instantiations: S slot registers per table part and lane, NP parts, NW waves per
-// workgroup so that the S*NP slots + ~50 working VGPRs fit NW/4 waves per SIMD (512
-// registers per SIMD lane), and L = ceil(kChunks / NW) direct-to-LDS loads per wave per
-// part.
-constexpr int nw_of(int P, int NP) {
-  return NP <= 2 ? (P <= 72 ? 16 : P <= 112 ? 12 : 8) : (P <= 48 ? 16 : P <= 96 ? 12 : 8);
+// workgroup so that the S*NP slots + ~30 working VGPRs fit NW/4 waves per SIMD (512
+// registers per SIMD lane: 128 per wave at 16 waves, 170 at 12, 256 at 8), and
+// L = ceil(kChunks / NW) direct-to-LDS loads per wave per part.
+constexpr int nw_of(int P, int NP, int X) {
+  // 16 waves need <= 128 VGPRs: up to 96 slots for whole frames (one 8-byte value of the
+  // epilogue spills, outside the hypothesis loop), up to 80 for split frames (X > 1: the
+  // lane-group reduction's registers would push slot addresses to scratch inside the loop)
+  const int p16 = X == 1 ? 96 : 80;
+  return P <= p16 ? 16 : (NP <= 2 ? (P <= 112 ? 12 : 8) : (P <= 96 ? 12 : 8));
 }
-constexpr int l_of(int P, int NP) { return (kChunks + nw_of(P, NP) - 1) / nw_of(P, NP); }
+constexpr int l_of(int P, int NP, int X) {
+  return (kChunks + nw_of(P, NP, X) - 1) / nw_of(P, NP, X);
+}
 
 struct GatherCfg {
   int S, NP, NW, X;
@@ -705,9 +723,11 @@ struct GatherCfg {
 };
 
 #define RSP_CFG(S, NP) \
-  {S, NP, nw_of(S * NP, NP), 1, &k_gather<S, NP, nw_of(S * NP, NP), l_of(S * NP, NP), 1>}
+  {S, NP, nw_of(S * NP, NP, 1), 1,  \
+   &k_gather<S, NP, nw_of(S * NP, NP, 1), l_of(S * NP, NP, 1), 1>}
 #define RSP_CFGX(S, NP, X) \
-  {S, NP, nw_of(S * NP, NP), X, &k_gather<S, NP, nw_of(S * NP, NP), l_of(S * NP, NP), X>}
+  {S, NP, nw_of(S * NP, NP, X), X,  \
+   &k_gather<S, NP, nw_of(S * NP, NP, X), l_of(S * NP, NP, X), X>}
 const GatherCfg kGatherCfgs[] = {
     RSP_CFG(16, 1), RSP_CFG(32, 1), RSP_CFG(48, 1), RSP_CFG(64, 1), RSP_CFG(80, 1),
     RSP_CFG(96, 1), RSP_CFG(112, 1), RSP_CFG(128, 1), RSP_CFG(160, 1),

@@ -221,15 +221,26 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       for (int i = 0; i < n; ++i) {
         const Entry E = sh[i];
         const int K = (int)E.k;
+        if (K == 1) {  // most entries: straight-line code over the candidates
 #pragma unroll
-        for (int u = 0; u < kPerThread; ++u) {
-          const double re = cr[u] + E.rr, im = ci[u] + E.ri;
-          const double a = re * re + im * im;
-          double pr = prod[u] * a;
-          for (int k = 1; k < K; ++k) pr = pr * a;
-          prod[u] = pr;
+          for (int u = 0; u < kPerThread; ++u) {
+            const double re = cr[u] + E.rr, im = ci[u] + E.ri;
+            prod[u] = prod[u] * (re * re + im * im);
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < kPerThread; ++u) {
+            const double re = cr[u] + E.rr, im = ci[u] + E.ri;
+            const double a = re * re + im * im;
+            double pr = prod[u] * a;
+            for (int k = 1; k < K; ++k) pr = pr * a;
+            prod[u] = pr;
+          }
         }
-        if ((i & 3) == 3 || K > 1) {
+        // renormalise every 8 factors (scaling by powers of two is exact, so the period
+        // does not change the product's rounding while 8 factors stay within FP64's
+        // normal range: |f_c + R_d|^2 in [1e-38, 1e38])
+        if ((i & 7) == 7 || K > 1) {
 #pragma unroll
           for (int u = 0; u < kPerThread; ++u) {
             int e;
