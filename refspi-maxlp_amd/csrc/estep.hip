@@ -183,22 +183,28 @@ __global__ __launch_bounds__(256) void k_tables(
 
 // Straight-line walk over the slot registers [K0, K1) in blocks of 8: every a[k] is a
 // statically named VGPR and, with no branches between blocks, the LDS reads of one block
-// overlap the arithmetic of the previous one.  Each block is summed pairwise in FP32
-// (error <= 3 * 2^-24 * sum|v|, accounted for in the per-frame bound of k_resolve) and
-// added to an FP64 accumulator.
+// overlap the arithmetic of the previous one.  Each block (8 slots, or 4 at the end of a
+// part of 8m+4 slots) is summed pairwise in FP32 (error <= 3 * 2^-24 * sum|v|, accounted
+// for in the per-frame bound of k_resolve) and added to an FP64 accumulator.
 // rb == kRebase for reads of the third buffer (one add per read: its base exceeds the
 // 16-bit immediate offset), else 0 (unused)
-template <int K, int P, bool REB>
-__device__ __forceinline__ void load8(const char* lds, const uint32_t (&a)[P], float (&v)[8],
-                                      uint32_t rb) {
+template <int K, int N, int P, bool REB>
+__device__ __forceinline__ void load_block(const char* lds, const uint32_t (&a)[P],
+                                           float (&v)[8], uint32_t rb) {
   if constexpr (REB) {
     const char* base = lds - kRebase;  // within the immediate offset
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(base + (a[K + u] + rb));
+    for (int u = 0; u < N; ++u) v[u] = *reinterpret_cast<const float*>(base + (a[K + u] + rb));
   } else {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K + u]);
+    for (int u = 0; u < N; ++u) v[u] = *reinterpret_cast<const float*>(lds + a[K + u]);
   }
+}
+
+// slots in the block starting at K of the range ending at K1 (8, or the 4 of a tail)
+template <int K, int K1>
+constexpr int block_len() {
+  return K1 - K >= 8 ? 8 : K1 - K;
 }
 
 // Software pipeline: the LDS reads of block K+8 are issued before the sums of block K, so
@@ -209,12 +215,18 @@ template <int K, int K1, int P, int B, bool REB>
 __device__ __forceinline__ void gather_pipe(const char* lds, const uint32_t (&a)[P],
                                             float (&v)[2][8], double (&acc)[4], uint32_t rb) {
   if constexpr (K < K1) {
-    if constexpr (K + 8 < K1) load8<K + 8, P, REB>(lds, a, v[B ^ 1], rb);
+    constexpr int N = block_len<K, K1>();
+    static_assert(N == 8 || N == 4, "parts hold a multiple of 4 slots");
+    if constexpr (K + N < K1) load_block<K + N, block_len<K + N, K1>(), P, REB>(lds, a, v[B ^ 1], rb);
     const float* w = v[B];
-    float s8 = ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+    float s8;
+    if constexpr (N == 8)
+      s8 = ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+    else
+      s8 = (w[0] + w[1]) + (w[2] + w[3]);
     asm volatile("" : "+v"(s8) : : "memory");
     acc[(K >> 3) & 3] += (double)s8;
-    gather_pipe<K + 8, K1, P, B ^ 1, REB>(lds, a, v, acc, rb);
+    gather_pipe<K + N, K1, P, B ^ 1, REB>(lds, a, v, acc, rb);
   }
 }
 
@@ -227,7 +239,7 @@ __device__ __forceinline__ void gather_range(const char* lds, const uint32_t (&a
     uint32_t rb = 0;
     if constexpr (REB) asm volatile("" : "=s"(rb) : "0"(kRebase));
     float v[2][8];
-    load8<K0, P, REB>(lds, a, v[0], rb);
+    load_block<K0, block_len<K0, K1>(), P, REB>(lds, a, v[0], rb);
     gather_pipe<K0, K1, P, 0, REB>(lds, a, v, acc, rb);
   }
 }
@@ -243,10 +255,10 @@ __device__ __forceinline__ void load_slots(uint32_t (&a)[P], const uint32_t* __r
 // --------------------------------------------------------------------------------
 // 3b: the gather.  Frames are processed in "sorted order" j (by scored-photon count);
 // wave gw holds frames j = 64*gw + lane.  slots[(gw*P + k)*64 + lane] is the LDS byte
-// address (within the table part resident in LDS) of the frame's k-th photon entry;
-// wave_meta[gw*kNPMax + p] is the end slot of part p (a multiple of 8).
+// address (within the table part resident in LDS) of the frame's k-th photon entry; slots
+// p*S .. p*S+S-1 belong to part p (S a multiple of 4).
 // --------------------------------------------------------------------------------
-// The two LDS part buffers of the gather, as separate objects: the compiler then knows
+// The three LDS part buffers of the gather, as separate objects: the compiler then knows
 // which buffer each direct-to-LDS load writes and waits only for the buffer it reads.
 __shared__ float g_buf0[kPStride], g_buf1[kPStride], g_buf2[kPStride];
 
@@ -711,7 +723,7 @@ constexpr int nw_of(int P, int NP, int X) {
   // epilogue spills, outside the hypothesis loop), up to 80 for split frames (X > 1: the
   // lane-group reduction's registers would push slot addresses to scratch inside the loop)
   const int p16 = X == 1 ? 96 : 80;
-  return P <= p16 ? 16 : (NP <= 2 ? (P <= 112 ? 12 : 8) : (P <= 96 ? 12 : 8));
+  return P <= p16 ? 16 : (NP <= 2 ? (P <= 120 ? 12 : 8) : (P <= 96 ? 12 : 8));
 }
 constexpr int l_of(int P, int NP, int X) {
   return (kChunks + nw_of(P, NP, X) - 1) / nw_of(P, NP, X);
@@ -731,10 +743,12 @@ struct GatherCfg {
 const GatherCfg kGatherCfgs[] = {
     RSP_CFG(16, 1), RSP_CFG(32, 1), RSP_CFG(48, 1), RSP_CFG(64, 1), RSP_CFG(80, 1),
     RSP_CFG(96, 1), RSP_CFG(112, 1), RSP_CFG(128, 1), RSP_CFG(160, 1),
-    RSP_CFG(8, 2),  RSP_CFG(16, 2), RSP_CFG(24, 2), RSP_CFG(32, 2), RSP_CFG(40, 2),
-    RSP_CFG(48, 2), RSP_CFG(56, 2), RSP_CFG(64, 2),
-    RSP_CFG(8, 3),  RSP_CFG(16, 3), RSP_CFG(24, 3), RSP_CFG(32, 3),
-    RSP_CFG(8, 4),  RSP_CFG(16, 4), RSP_CFG(24, 4),
+    RSP_CFG(8, 2),  RSP_CFG(16, 2), RSP_CFG(24, 2), RSP_CFG(32, 2), RSP_CFG(36, 2),
+    RSP_CFG(40, 2), RSP_CFG(44, 2), RSP_CFG(48, 2), RSP_CFG(52, 2), RSP_CFG(56, 2),
+    RSP_CFG(60, 2), RSP_CFG(64, 2),
+    RSP_CFG(8, 3),  RSP_CFG(16, 3), RSP_CFG(20, 3), RSP_CFG(24, 3), RSP_CFG(28, 3),
+    RSP_CFG(32, 3),
+    RSP_CFG(8, 4),  RSP_CFG(12, 4), RSP_CFG(16, 4), RSP_CFG(20, 4), RSP_CFG(24, 4),
     // frames split over X lanes
     RSP_CFGX(96, 1, 2), RSP_CFGX(96, 1, 4), RSP_CFGX(96, 1, 8),
     RSP_CFGX(40, 2, 2), RSP_CFGX(40, 2, 4), RSP_CFGX(40, 2, 8),
@@ -929,7 +943,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   struct Wave {
     int X;
     int64_t f0, nf;  // frames order[f0 .. f0+nf)
-    int need;        // slots per part (max over parts, multiple of 8)
+    int need;        // slots per part (max over parts, multiple of 4)
     std::vector<std::vector<std::vector<int>>> placed;  // [part][lane][slot]
   };
   std::vector<Wave> waves;
@@ -964,7 +978,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
           for (size_t r = 0; r < ent[l].size(); ++r) pl[slot_of[l][r]] = ent[l][r];
         }
       }
-      wv.need = std::max(wv.need, (int)ceil_div(ns_max, 8) * 8);
+      wv.need = std::max(wv.need, (int)ceil_div(ns_max, 4) * 4);
     }
   };
   for (int64_t f0 = 0; f0 < N;) {
