@@ -374,6 +374,7 @@ def main():
     torch.cuda.synchronize()
     gather_ms = tables_ms = 0.
     batches = 0
+    lds_rw = 0.
     t0 = time.time()
     for i in range(args.steps):
         ts = time.time()
@@ -382,6 +383,7 @@ def main():
         gather_ms += tm['gather_ms']
         tables_ms += tm['tables_ms']
         batches += tm['batches']
+        lds_rw += H * (tm['lds_read_bytes_per_hyp'] + tm['lds_write_bytes_per_hyp'])
         log('step %d: %.2f s %s gather %.1f ms tables %.1f ms' %
             (i, time.time() - ts, emc.timings, tm['gather_ms'], tm['tables_ms']))
     torch.cuda.synchronize()
@@ -404,6 +406,9 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get('bytes_per_launch')
+    avg_launch_s = gather_s / max(batches, 1)
+    hbm_gbs = traffic / avg_launch_s / 1e9 if traffic else None
+    lds_exec_gbs = lds_rw / gather_s / 1e9
     out = dict(
         metric='frame x rotation evals/s per EMC+MaxLP iteration at 185^2 (x states, joint '
                'search)',
@@ -420,11 +425,19 @@ def main():
                       frac=achieved / HBM_PEAK_GBS, traffic=traffic,
                       kernel='k_gather', launches=batches,
                       avg_launch_ms=gather_ms / max(batches, 1),
-                      note='algorithmic bytes = H x sum_d(4 ones_d + 8 multi_d) per step; '
-                           'photons stay in VGPRs across hypotheses so this exceeds HBM '
-                           'traffic; the binding resource is LDS (see lds)'),
+                      hbm_measured_gbs=hbm_gbs,
+                      hbm_measured_frac=hbm_gbs / HBM_PEAK_GBS if hbm_gbs else None,
+                      note='algorithmic bytes = H x sum_d(4 ones_d + 8 multi_d) per step (the '
+                           'sparse records calc_prob_all re-reads per hypothesis); here the '
+                           'photons stay in VGPRs across hypotheses, so achieved/peak is not '
+                           'an HBM utilisation: the HBM rate actually drawn is '
+                           'hbm_measured_gbs (traffic / avg launch), and the binding resource '
+                           'is LDS (see lds)'),
         lds=dict(achieved=lds_gbs, peak=LDS_PEAK_GBS, unit='GB/s', frac=lds_gbs / LDS_PEAK_GBS,
-                 what='scored-photon gathers (4 B each) per step / gather time'),
+                 what='scored-photon gathers (4 B each) per step / gather time',
+                 executed_gbs=lds_exec_gbs, executed_frac=lds_exec_gbs / LDS_PEAK_GBS,
+                 executed_what='LDS bytes the gather executes: every slot read (bank-schedule '
+                               'zero pads included) + the table loads\' LDS writes'),
     )
     if args.workload == 'all':
         del emc

@@ -102,9 +102,10 @@ int rsp_emc_estep(rsp_emc* plan, const double* model_c, const double* scales,
  * [6] waves per workgroup, [7] table parts. */
 int rsp_emc_stats(const rsp_emc* plan, int64_t* out);
 
-/* Timing of the last E-step with option 5 on (host double[4]): [0] total ms of the
+/* Timing of the last E-step with option 5 on (host double[6]): [0] total ms of the
  * table kernels, [1] total ms of the gather kernels, [2] batches (= launches of each),
- * [3] hypotheses per batch. */
+ * [3] hypotheses per batch; from the plan: [4] LDS bytes the gather reads per hypothesis
+ * (every slot, zero pads included), [5] LDS bytes its table loads write per hypothesis. */
 int rsp_emc_timing(const rsp_emc* plan, double* out);
 
 int rsp_emc_destroy(rsp_emc* plan);

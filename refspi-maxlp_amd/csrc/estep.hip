@@ -1250,6 +1250,18 @@ extern "C" int rsp_emc_timing(const rsp_emc* e, double* out) {
   out[1] = e->t_gather_ms;
   out[2] = (double)e->n_batches;
   out[3] = (double)e->batch;
+  // LDS traffic of the gather per hypothesis, from the plan's slot classes: every slot
+  // register is one 4-byte read per lane (zero pads included); every tile loads NW * L
+  // 1 KB chunks per part (surplus loads included)
+  double rd = 0., wr = 0.;
+  for (const auto& c : e->classes) {
+    const GatherCfg& g = kGatherCfgs[c.cfg];
+    const int L = (kChunks + g.NW - 1) / g.NW;
+    rd += (double)c.n_tiles * g.NW * 64. * g.S * g.NP * 4.;
+    wr += (double)c.n_tiles * g.NP * g.NW * L * 1024.;
+  }
+  out[4] = rd;
+  out[5] = wr;
   return kOk;
 }
 

@@ -232,10 +232,11 @@ class EMC:
         return dict(zip(keys, (int(v) for v in out)))
 
     def estep_timing(self):
-        out = np.zeros(4, dtype='f8')
+        out = np.zeros(6, dtype='f8')
         _lib.call('rsp_emc_timing', self._plan, _lib.host_ptr(out, C.c_double))
         return dict(tables_ms=float(out[0]), gather_ms=float(out[1]), batches=int(out[2]),
-                    batch=int(out[3]))
+                    batch=int(out[3]), lds_read_bytes_per_hyp=float(out[4]),
+                    lds_write_bytes_per_hyp=float(out[5]))
 
     # -- emc.py:207-216 (one rank)
     def _get_rmax(self):
