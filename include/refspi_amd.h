@@ -85,7 +85,10 @@ int rsp_emc_set_frames(rsp_emc* plan, int64_t num_data, const int32_t* ones,
  *   3 = LDS bank model of the photon scheduler (32 = conflict-free half-waves, the
  *       default and gfx950's layout; 64; 0 = no scheduling; set before frames)
  *   4 = hypothesis segments per batch (default automatic)
- *   5 = per-kernel timing with HIP events (1 on; read with rsp_emc_timing) */
+ *   5 = per-kernel timing with HIP events (1 on; read with rsp_emc_timing)
+ *   6 = swaps of the local search that regroups frames into lane groups with flat LDS
+ *       bank loads, per window of 8 groups (default 4000; 0 = off; set before frames).
+ *       Only the slot layout changes, never the result. */
 int rsp_emc_set_option(rsp_emc* plan, int key, int64_t value);
 
 /* Run the E-step for `model_c` (device, npix complex) with per-frame `scales` (device,

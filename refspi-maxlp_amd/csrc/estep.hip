@@ -618,6 +618,7 @@ struct rsp_emc {
   // options / stats
   int exact_mode = 0;
   int64_t opt_batch = 0, opt_nseg = 0;
+  int regroup_iters = 4000;  // local-search swaps per window of 8 lane groups (0: off)
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   struct Class {
     int cfg, n_tiles, gw0;
@@ -670,6 +671,100 @@ void free_all(rsp_emc* e) {
 // part.  entries[l] lists LDS entry indices (within the part) for lane l; the result
 // places every entry of lane l in a distinct slot so that, within the group, no two
 // lanes read the same bank in the same slot.  Returns the number of slots used.
+// Regrouping of frames into lane groups before the waves are built.  The slot count of a
+// group of G lanes (a half-wave for 32 banks) on one part is at least
+// delta = max(largest per-part entry count of its frames, busiest bank of its entries),
+// and the greedy schedule of schedule_group stays within a fraction of a slot of it.  With
+// frames in descending order of entry count, the lane term is nearly flat over a window of
+// similar frames, but the bank term is a sum of G frames' bank histograms and sits well
+// above its mean.  Within a window of ng groups this randomised local search swaps frames
+// between groups -- always out of the group with the largest delta, the frame that loads
+// its busiest (part, bank) cell the most -- and keeps a swap when the larger of the two
+// groups' deltas falls, or stays and their sum falls.  The groups are then emitted in
+// descending order of delta, so that the two halves of a wave are alike.  The seed is
+// fixed: the same frames give the same plan.
+struct FrameHist {
+  int NP = 0, NB = 0;
+  std::vector<uint16_t> h;    // [frame][part][bank] entry counts
+  std::vector<int32_t> pc;    // [frame][part] entry counts
+  const uint16_t* hist(int32_t d) const { return h.data() + (size_t)d * NP * NB; }
+  const int32_t* parts(int32_t d) const { return pc.data() + (size_t)d * NP; }
+};
+
+void regroup_window(std::vector<int32_t>& order, int64_t f0, int64_t f1, int G,
+                    const FrameHist& fh, int iters, uint64_t seed) {
+  const int ng = (int)((f1 - f0) / G);
+  if (ng < 2) return;
+  const int K = fh.NP * fh.NB;
+  std::vector<int32_t> gf(order.begin() + f0, order.begin() + f0 + (int64_t)ng * G);
+  std::vector<int> load((size_t)ng * K, 0);
+  for (int g = 0; g < ng; ++g)
+    for (int i = 0; i < G; ++i) {
+      const uint16_t* h = fh.hist(gf[(size_t)g * G + i]);
+      for (int k = 0; k < K; ++k) load[(size_t)g * K + k] += h[k];
+    }
+  // delta of group g with frame slot i holding frame `x` instead (i < 0: as it is)
+  auto lane_max = [&](int g, int i, int32_t x) {
+    int m = 0;
+    for (int j = 0; j < G; ++j) {
+      const int32_t* p = fh.parts(j == i ? x : gf[(size_t)g * G + j]);
+      for (int q = 0; q < fh.NP; ++q) m = std::max(m, p[q]);
+    }
+    return m;
+  };
+  std::vector<int> delta(ng);
+  for (int g = 0; g < ng; ++g)
+    delta[g] = std::max(*std::max_element(load.begin() + (size_t)g * K,
+                                          load.begin() + (size_t)(g + 1) * K),
+                        lane_max(g, -1, 0));
+  uint64_t s = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto rnd = [&](uint64_t n) {  // splitmix64
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (z ^ (z >> 31)) % n;
+  };
+  std::vector<int> la(K), lb(K);
+  for (int it = 0; it < iters; ++it) {
+    const int a = (int)(std::max_element(delta.begin(), delta.end()) - delta.begin());
+    const int* La = load.data() + (size_t)a * K;
+    const int cell = (int)(std::max_element(La, La + K) - La);
+    int ia = 0;
+    for (int i = 1; i < G; ++i)
+      if (fh.hist(gf[(size_t)a * G + i])[cell] > fh.hist(gf[(size_t)a * G + ia])[cell]) ia = i;
+    int b = (int)rnd(ng - 1);
+    b += b >= a;
+    const int ib = (int)rnd(G);
+    const int32_t x = gf[(size_t)a * G + ia], y = gf[(size_t)b * G + ib];
+    const uint16_t *hx = fh.hist(x), *hy = fh.hist(y);
+    const int* Lb = load.data() + (size_t)b * K;
+    int ma = 0, mb = 0;
+    for (int k = 0; k < K; ++k) {
+      la[k] = La[k] - hx[k] + hy[k];
+      lb[k] = Lb[k] - hy[k] + hx[k];
+      ma = std::max(ma, la[k]);
+      mb = std::max(mb, lb[k]);
+    }
+    const int na = std::max(ma, lane_max(a, ia, y)), nb = std::max(mb, lane_max(b, ib, x));
+    const int hi = std::max(na, nb);
+    if (hi < delta[a] || (hi == delta[a] && na + nb < delta[a] + delta[b])) {
+      std::copy(la.begin(), la.end(), load.begin() + (size_t)a * K);
+      std::copy(lb.begin(), lb.end(), load.begin() + (size_t)b * K);
+      gf[(size_t)a * G + ia] = y;
+      gf[(size_t)b * G + ib] = x;
+      delta[a] = na;
+      delta[b] = nb;
+    }
+  }
+  std::vector<int> gord(ng);
+  std::iota(gord.begin(), gord.end(), 0);
+  std::stable_sort(gord.begin(), gord.end(), [&](int p, int q) { return delta[p] > delta[q]; });
+  for (int r = 0; r < ng; ++r)
+    std::copy(gf.begin() + (size_t)gord[r] * G, gf.begin() + (size_t)(gord[r] + 1) * G,
+              order.begin() + f0 + (int64_t)r * G);
+}
+
 int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
                    std::vector<std::vector<int>>& slot_of) {
   const int G = (int)entries.size();
@@ -859,6 +954,10 @@ extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
       e->opt_nseg = value;
       return kOk;
     case 5: e->timing = value != 0; return kOk;
+    case 6:
+      if (value < 0 || value > (1 << 24)) return fail(kErrSize, "regroup iterations out of range");
+      e->regroup_iters = (int)value;
+      return kOk;
     default: return fail(kErrSize, "rsp_emc_set_option: unknown key");
   }
 }
@@ -981,8 +1080,35 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
       wv.need = std::max(wv.need, (int)ceil_div(ns_max, 4) * 4);
     }
   };
+  // per-frame (part, bank) histograms for the regrouping of whole-frame waves
+  FrameHist fh;
+  const bool regroup = e->regroup_iters > 0 && NB > 0;
+  if (regroup) {
+    fh.NP = e->NP;
+    fh.NB = NB;
+    fh.h.assign((size_t)N * e->NP * NB, 0);
+    fh.pc.assign((size_t)N * e->NP, 0);
+    for (int64_t d = 0; d < N; ++d)
+      for (int64_t q = ex_off[d]; q < ex_off[d + 1]; ++q) {
+        const int ci = e->compact_of[ex_pix[q]];
+        const int c = (q < ex_off[d] + ex_n1[d]) ? 1 : ex_cnt[q];
+        const int p = ci / kPartMax, bank = (ci - p * kPartMax) % NB;
+        uint16_t& hv = fh.h[((size_t)d * e->NP + p) * NB + bank];
+        hv = (uint16_t)std::min(65535, (int)hv + c);
+        fh.pc[(size_t)d * e->NP + p] += c;
+      }
+  }
+  int64_t next_window = 0;
   for (int64_t f0 = 0; f0 < N;) {
     Wave wv;
+    if (regroup && f0 >= next_window) {
+      // regroup the next 8 lane groups once the waves hold whole frames (X = 1)
+      build_wave(f0, 1, wv);
+      if (wv.need <= max_slots(e->NP, 1)) {
+        next_window = std::min<int64_t>(N, f0 + 8 * (int64_t)G);
+        regroup_window(order, f0, next_window, G, fh, e->regroup_iters, (uint64_t)f0);
+      }
+    }
     int X = 1;
     for (;; X *= 2) {
       build_wave(f0, X, wv);

@@ -67,7 +67,7 @@ class EMC:
     def __init__(self, config_file=None, num_streams=4, *, dataset=None, size=None,
                  num_rot=None, shiftx=None, shifty=None, sphere_dia=None, need_scaling=None,
                  output_folder=None, model=None, seed=None, device=0, save=True,
-                 bank_count=None):
+                 bank_count=None, regroup_iters=None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('EMC needs a GPU (HIP device)')
@@ -76,6 +76,7 @@ class EMC:
         self.rank, self.num_proc = 0, 1
         self.save = save
         self.bank_count = bank_count
+        self.regroup_iters = regroup_iters
         config = configparser.ConfigParser()
         start_dir = '.'
         if config_file is not None:
@@ -165,6 +166,9 @@ class EMC:
                                       len(self.shiftx), C.byref(h)), 'rsp_emc_create')
         if self.bank_count is not None:   # LDS bank model of the photon scheduler
             _lib.check(lib.rsp_emc_set_option(h, 3, int(self.bank_count)), 'rsp_emc_set_option')
+        if self.regroup_iters is not None:   # swaps of the frame regrouping per window
+            _lib.check(lib.rsp_emc_set_option(h, 6, int(self.regroup_iters)),
+                       'rsp_emc_set_option')
         d = self.dset
         bg = np.ascontiguousarray(d.bg, dtype='f8')
         _lib.check(lib.rsp_emc_set_frames(

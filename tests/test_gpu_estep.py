@@ -125,6 +125,20 @@ def test_estep_bank_models_do_not_change_result(small_frames, small_model, banks
     np.testing.assert_array_equal(emc.maxll, base.maxll)
 
 
+def test_estep_frame_regrouping_does_not_change_result(small_frames, small_model):
+    """Option 6 (swaps of the lane-group regrouping) changes only the slot layout."""
+    import torch
+    m = torch.as_tensor(small_model, device='cuda')
+    out = []
+    for iters in (0, 50, 4000):
+        emc = _emc(small_frames, small_model, regroup_iters=iters)
+        emc._calculate_prob(m)
+        out.append((emc._get_rmax().copy(), emc.maxll.copy()))
+    for r, l in out[1:]:
+        assert np.array_equal(r, out[0][0])
+        np.testing.assert_array_equal(l, out[0][1])
+
+
 def test_estep_batching_and_segments_do_not_change_result(small_frames, small_model):
     import torch
     m = torch.as_tensor(small_model, device='cuda')
