@@ -281,20 +281,33 @@ struct GatherStep {
   int TS, hb, h0, h1, nq, R, wid, lane;
   double scale;
 
-  // Loads of step q into dst.  Steps past the end re-load the first part (in bounds) into
-  // a buffer that is never read for a real hypothesis: every step issues exactly L loads,
-  // so the wait counts are the same on every path and the compiler adds no waits.
+  // Loads of step q into dst: chunks 0 .. kChunks-2 as 16-byte-per-lane loads, L - 1 per
+  // wave (chunk c from wave c % NW), and the last chunk as 4-byte-per-lane loads, one per
+  // wave: wave w loads quarter w % 4 of it (waves 4.. repeat a quarter with the same bytes,
+  // 256 B each instead of a whole 1 KB chunk).  Every wave issues exactly L loads on every
+  // path -- steps past the end re-load the first part (in bounds) into a buffer that is
+  // never read for a real hypothesis -- so the wait counts are the same everywhere and
+  // the compiler adds no waits.
+  static_assert((L - 1) * NW == kChunks - 1, "chunks split as (L - 1) per wave + 1");
   __device__ __forceinline__ void issue(int q, float* dst) const {
     if (q >= nq) q = 0;
     const int h = h0 + q / NP, p = q - (q / NP) * NP;
     const float* src = tables + (int64_t)(h - hb) * TS + (int64_t)p * kPStride;
+    // addresses as a wave-uniform base + one 32-bit lane offset (scalar-base loads: no
+    // per-load address registers)
+    const char* sb = reinterpret_cast<const char*>(src);
+    const unsigned l16 = (unsigned)lane * 16u, l4 = (unsigned)lane * 4u;
 #pragma unroll
-    for (int u = 0; u < L; ++u) {
-      const int c = min(wid + u * NW, kChunks - 1);
+    for (int u = 0; u < L - 1; ++u) {
+      const int c = wid + u * NW;
       __builtin_amdgcn_global_load_lds(
-          (const __attribute__((address_space(1))) void*)(src + (c * 64 + lane) * 4),
+          (const __attribute__((address_space(1))) void*)(sb + c * 1024 + l16),
           (__attribute__((address_space(3))) void*)(dst + c * 256), 16, 0, 0);
     }
+    const int qo = (kChunks - 1) * 256 + (wid & 3) * 64;
+    __builtin_amdgcn_global_load_lds(
+        (const __attribute__((address_space(1))) void*)(sb + qo * 4 + l4),
+        (__attribute__((address_space(3))) void*)(dst + qo), 4, 0, 0);
   }
 
   // static step QS (0 .. 3*NP-1) of group g
@@ -374,8 +387,8 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   uint32_t a[P];
   load_slots<0, P>(a, slots + (int64_t)lw * P * 64 + lane);
   Top3 top = top3_empty();
-  // every wave issues exactly L 1 KB direct-to-LDS loads per step (surplus loads repeat the
-  // last chunk with identical bytes); two steps are in flight ahead of the gather
+  // every wave issues exactly L direct-to-LDS loads per step; two steps are in flight
+  // ahead of the gather
   const GatherStep<S, NP, NW, L, X> st{tables, TS, hb, h0, h1, (h1 - h0) * NP, R, wid, lane,
                                     scale_sorted[j]};
   st.issue(0, g_buf0);
