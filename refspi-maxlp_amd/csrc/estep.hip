@@ -184,8 +184,9 @@ __global__ __launch_bounds__(256) void k_tables(
 // Straight-line walk over the slot registers [K0, K1) in blocks of 8: every a[k] is a
 // statically named VGPR and, with no branches between blocks, the LDS reads of one block
 // overlap the arithmetic of the previous one.  Each block (8 slots, or 4 at the end of a
-// part of 8m+4 slots) is summed pairwise in FP32 (error <= 3 * 2^-24 * sum|v|, accounted
-// for in the per-frame bound of k_resolve) and added to an FP64 accumulator.
+// part of 8m+4 slots) is summed pairwise in FP32; pairs of blocks are added in FP32 (a
+// 16-term pairwise sum: error <= gamma_4 * sum|v|, gamma_4 = 4u/(1-4u), u = 2^-24,
+// accounted for in the per-frame bound of k_resolve) and then to an FP64 accumulator.
 // rb == kRebase for reads of the third buffer (one add per read: its base exceeds the
 // 16-bit immediate offset), else 0 (unused)
 template <int K, int N, int P, bool REB>
@@ -211,9 +212,11 @@ constexpr int block_len() {
 // every wave keeps 8 reads in flight while it adds instead of waiting for all of a part's
 // reads at once.  The empty asm statements pin the order: each block's sum is complete
 // before the next block's reads are issued (memory clobber), and cannot sink below them.
-template <int K, int K1, int P, int B, bool REB>
+// PEND: the FP32 sum of the previous block is waiting in `pend` for its pair.
+template <int K, int K1, int P, int B, bool REB, bool PEND>
 __device__ __forceinline__ void gather_pipe(const char* lds, const uint32_t (&a)[P],
-                                            float (&v)[2][8], double (&acc)[4], uint32_t rb) {
+                                            float (&v)[2][8], double (&acc)[4], uint32_t rb,
+                                            float pend) {
   if constexpr (K < K1) {
     constexpr int N = block_len<K, K1>();
     static_assert(N == 8 || N == 4, "parts hold a multiple of 4 slots");
@@ -225,8 +228,14 @@ __device__ __forceinline__ void gather_pipe(const char* lds, const uint32_t (&a)
     else
       s8 = (w[0] + w[1]) + (w[2] + w[3]);
     asm volatile("" : "+v"(s8) : : "memory");
-    acc[(K >> 3) & 3] += (double)s8;
-    gather_pipe<K + N, K1, P, B ^ 1, REB>(lds, a, v, acc, rb);
+    if constexpr (PEND) {
+      acc[(K >> 4) & 3] += (double)(pend + s8);
+      gather_pipe<K + N, K1, P, B ^ 1, REB, false>(lds, a, v, acc, rb, 0.f);
+    } else if constexpr (K + N >= K1) {
+      acc[(K >> 4) & 3] += (double)s8;  // last block of the part, unpaired
+    } else {
+      gather_pipe<K + N, K1, P, B ^ 1, REB, true>(lds, a, v, acc, rb, s8);
+    }
   }
 }
 
@@ -240,7 +249,7 @@ __device__ __forceinline__ void gather_range(const char* lds, const uint32_t (&a
     if constexpr (REB) asm volatile("" : "=s"(rb) : "0"(kRebase));
     float v[2][8];
     load_block<K0, block_len<K0, K1>(), P, REB>(lds, a, v[0], rb);
-    gather_pipe<K0, K1, P, 0, REB>(lds, a, v, acc, rb);
+    gather_pipe<K0, K1, P, 0, REB, false>(lds, a, v, acc, rb, 0.f);
   }
 }
 
@@ -493,9 +502,10 @@ __global__ __launch_bounds__(256) void k_resolve(ExactArgs A, const Top3* __rest
   for (int64_t e = A.ex_off[d]; e < A.ex_off[d + 1]; ++e) {
     const int c = (e < A.ex_off[d] + A.ex_n1[d]) ? 1 : A.ex_cnt[e];
     const int ci = compact_of[A.ex_pix[e]];
-    // table rounding + the FP32 pairwise 8-sums of the gather (gamma_3 * |value|)
+    // table rounding + the FP32 pairwise 16-sums of the gather (gamma_4 * |value|,
+    // gamma_4 = 4u/(1-4u) = 2.384e-7 for u = 2^-24)
     E += (double)c * ((double)__uint_as_float(errmax[ci]) +
-                      1.8e-7 * (double)__uint_as_float(absmax[ci]));
+                      2.4e-7 * (double)__uint_as_float(absmax[ci]));
   }
   // plus the FP64 summation error of both paths: n*2^-53*sum|terms| with |log-view| <= 100
   // (values of -1000 mark out-of-range samples and never win); 1000x safety factor
