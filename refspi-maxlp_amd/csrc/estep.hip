@@ -418,33 +418,49 @@ struct SegMap {
   int nseg[kMaxClass];
 };
 
+// Four lanes per frame: lane u of the group merges segments u, u+4, ...; the four partial
+// top-3 are then combined by shuffles.  The top two are selected by (value, lower index)
+// and v3 is a maximum, so the result does not depend on the order of the merges.
+__device__ __forceinline__ void top3_merge(Top3& t, const Top3& p) {
+  top3_insert(t, p.v1, p.h1);
+  top3_insert(t, p.v2, p.h2);
+  if (p.v3 > t.v3) t.v3 = p.v3;
+}
+
 __global__ __launch_bounds__(256) void k_merge_top3(const Top3* __restrict__ partials, SegMap sm,
                                                     int Npad, Top3* __restrict__ running,
                                                     int first) {
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= Npad) return;
+  const int tid = blockIdx.x * 256 + threadIdx.x;
+  const int j = tid >> 2, u = tid & 3;
+  const bool live = j < Npad;   // no early return: the shuffles below need all lanes
   const int gw = j >> 6;
   int nseg = 0;
   for (int c = sm.n - 1; c >= 0; --c)
     if (gw < sm.gw_end[c]) nseg = sm.nseg[c];
-  Top3 t = first ? top3_empty() : running[j];
-  // the partials of 8 segments are loaded before any is merged: up to ~200 segments per
-  // frame would otherwise cost one memory latency each
+  if (!live) nseg = 0;
+  Top3 t = (first || u != 0 || !live) ? top3_empty() : running[j];
+  // the partials of 8 of this lane's segments are loaded before any is merged
   constexpr int kU = 8;
-  for (int g0 = 0; g0 < nseg; g0 += kU) {
+  for (int g0 = u; g0 < nseg; g0 += 4 * kU) {
     Top3 p[kU];
 #pragma unroll
-    for (int u = 0; u < kU; ++u)
-      if (g0 + u < nseg) p[u] = partials[(int64_t)(g0 + u) * Npad + j];
+    for (int v = 0; v < kU; ++v)
+      if (g0 + 4 * v < nseg) p[v] = partials[(int64_t)(g0 + 4 * v) * Npad + j];
 #pragma unroll
-    for (int u = 0; u < kU; ++u)
-      if (g0 + u < nseg) {
-        top3_insert(t, p[u].v1, p[u].h1);
-        top3_insert(t, p[u].v2, p[u].h2);
-        if (p[u].v3 > t.v3) t.v3 = p[u].v3;
-      }
+    for (int v = 0; v < kU; ++v)
+      if (g0 + 4 * v < nseg) top3_merge(t, p[v]);
   }
-  running[j] = t;
+#pragma unroll
+  for (int off = 1; off < 4; off <<= 1) {
+    Top3 o;
+    o.v1 = __shfl_xor(t.v1, off);
+    o.v2 = __shfl_xor(t.v2, off);
+    o.v3 = __shfl_xor(t.v3, off);
+    o.h1 = __shfl_xor(t.h1, off);
+    o.h2 = __shfl_xor(t.h2, off);
+    top3_merge(t, o);
+  }
+  if (live && u == 0) running[j] = t;
 }
 
 // --------------------------------------------------------------------------------
@@ -1326,7 +1342,7 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
       }
       RSP_LAUNCH_CHECK("k_gather");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));
-      hipLaunchKernelGGL(k_merge_top3, dim3((unsigned)ceil_div(e->Npad, 256)), dim3(256), 0, st,
+      hipLaunchKernelGGL(k_merge_top3, dim3((unsigned)ceil_div(4 * e->Npad, 256)), dim3(256), 0, st,
                          e->d_partials, sm, (int)e->Npad, e->d_running, first);
       RSP_LAUNCH_CHECK("k_merge_top3");
       first = 0;
