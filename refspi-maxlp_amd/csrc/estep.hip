@@ -352,7 +352,14 @@ struct GatherStep {
       // the init term; every lane of the group then holds the same total)
 #pragma unroll
       for (int off = 1; off < X; off <<= 1) tot += __shfl_xor(tot, off);
-      if (h < h1) top3_insert(top, tot, h);
+      // below the third value nothing changes (v3 <= v2 <= v1): most hypotheses stop here,
+      // and a wave with no lane above its v3 skips the insertion (not for 96 slots: the
+      // branch's registers would push slot addresses to scratch)
+      if constexpr (P < 96) {
+        if (h < h1 && tot >= top.v3) top3_insert(top, tot, h);
+      } else {
+        if (h < h1) top3_insert(top, tot, h);
+      }
     }
   }
 
