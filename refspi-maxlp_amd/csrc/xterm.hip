@@ -9,10 +9,11 @@
 // with FP32 accumulation in v_mfma_f32_32x32x16_bf16 over K slices of 12288 (split-K); the
 // slices' partial products are added in FP64 by their consumer.
 //
-// Kernel layout (k_gemm_split): 256 threads = 4 waves, macro tile 256 (r) x 128 (d),
-// each wave 128 x 64 = 4 x 2 MFMA tiles of 32 x 32.  K advances in stages of 32 bf16
-// through a ring of three static LDS buffers (48 KB per stage) filled by direct
-// global->LDS loads two stages ahead.  LDS rows are 64 B (4 slots of 16 B) with the
+// Kernel layout (k_gemm_split): 256 threads = 4 waves, macro tile 256 (r) x 256 (d),
+// each wave 128 x 128 = 4 x 4 MFMA tiles of 32 x 32 (accumulators in AGPRs).  K advances
+// in stages of 32 bf16 through two static LDS buffers (64 KB per stage) filled by direct
+// global->LDS loads one stage ahead; within a stage the fragment reads of the second
+// 16-k step overlap the MFMAs of the first.  LDS rows are 64 B (4 slots of 16 B) with the
 // slot XOR-swizzled by f(row) = ((row >> 1) ^ (row >> 3)) & 3, which makes the
 // fragment reads (ds_read_b128, lane -> row lane%32, slot 2*kk + lane/32) hit distinct
 // banks for every 8-, 16- and 32-lane group.  Work-groups are mapped XCD-major so the
@@ -33,34 +34,32 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTM = 256, kTN = 128, kBK = 32;
+constexpr int kTM = 256, kTN = 256, kBK = 32;
 constexpr int kRowB = kBK * 2;                 // bytes per tile row (64)
 constexpr int kTileA = kTM * kRowB;            // 16 KB per A plane tile
-constexpr int kTileB = kTN * kRowB;            // 8 KB per B plane tile
+constexpr int kTileB = kTN * kRowB;            // 16 KB per B plane tile
 constexpr int kLA = kTileA / 1024 / 4;         // DMA instructions per wave per A plane (4)
-constexpr int kLB = kTileB / 1024 / 4;         // per B plane (2)
-constexpr int kL = 2 * (kLA + kLB);            // loads per wave per stage (12)
+constexpr int kLB = kTileB / 1024 / 4;         // per B plane (4)
 // K stages per split-K slice: 384 stages = 12288 K elements accumulated in FP32 per slice
 // (the slices are added in FP64 by the consumer).
 constexpr int kSliceStages = 384;
-static_assert(kL < 16, "vmcnt immediate");
 
-__shared__ __attribute__((aligned(16))) char s_ah0[kTileA], s_ah1[kTileA], s_ah2[kTileA];
-__shared__ __attribute__((aligned(16))) char s_al0[kTileA], s_al1[kTileA], s_al2[kTileA];
-__shared__ __attribute__((aligned(16))) char s_bh0[kTileB], s_bh1[kTileB], s_bh2[kTileB];
-__shared__ __attribute__((aligned(16))) char s_bl0[kTileB], s_bl1[kTileB], s_bl2[kTileB];
+__shared__ __attribute__((aligned(16))) char s_ah0[kTileA], s_ah1[kTileA];
+__shared__ __attribute__((aligned(16))) char s_al0[kTileA], s_al1[kTileA];
+__shared__ __attribute__((aligned(16))) char s_bh0[kTileB], s_bh1[kTileB];
+__shared__ __attribute__((aligned(16))) char s_bl0[kTileB], s_bl1[kTileB];
 
 template <int S> __device__ __forceinline__ char* t_ah() {
-  if constexpr (S == 0) return s_ah0; else if constexpr (S == 1) return s_ah1; else return s_ah2;
+  if constexpr (S == 0) return s_ah0; else return s_ah1;
 }
 template <int S> __device__ __forceinline__ char* t_al() {
-  if constexpr (S == 0) return s_al0; else if constexpr (S == 1) return s_al1; else return s_al2;
+  if constexpr (S == 0) return s_al0; else return s_al1;
 }
 template <int S> __device__ __forceinline__ char* t_bh() {
-  if constexpr (S == 0) return s_bh0; else if constexpr (S == 1) return s_bh1; else return s_bh2;
+  if constexpr (S == 0) return s_bh0; else return s_bh1;
 }
 template <int S> __device__ __forceinline__ char* t_bl() {
-  if constexpr (S == 0) return s_bl0; else if constexpr (S == 1) return s_bl1; else return s_bl2;
+  if constexpr (S == 0) return s_bl0; else return s_bl1;
 }
 
 __device__ __forceinline__ int swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
@@ -69,6 +68,12 @@ __device__ __forceinline__ void dma16(const void* g, char* l) {
   __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)g,
                                    (__attribute__((address_space(3))) void*)l, 16, 0, 0);
 }
+
+// The fragments of one 16-wide k step of a wave's 128 x 128 tile: rows of A (hi, lo) and
+// of B (hi, lo), four 32-row MFMA operands each.
+struct Frags {
+  bf16x8 ah[4], al[4], bh[4], bl[4];
+};
 
 struct SplitGemm {
   const __bf16 *ah, *al, *bh, *bl;   // planes: A [Mpad x Kp], B [Npad x Kp]
@@ -91,50 +96,76 @@ struct SplitGemm {
   }
 
   template <int S>
-  __device__ __forceinline__ void compute(f32x16 (&acc)[4][2], int wm, int wn) const {
+  __device__ __forceinline__ void load(Frags& f, int kk, int wm, int wn) const {
     const int lr = lane & 31;
-    const int rowoff = lr * kRowB;
-    const int fs = swz(lr);
+    const int kc = ((2 * kk + (lane >> 5)) ^ swz(lr)) * 16;
 #pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
-      const int kc = ((2 * kk + (lane >> 5)) ^ fs) * 16;
-      bf16x8 a_h[4], a_l[4], b_h[2], b_l[2];
+    for (int i = 0; i < 4; ++i) {
+      const int o = (wm * 128 + i * 32 + lr) * kRowB + kc;
+      f.ah[i] = *reinterpret_cast<const bf16x8*>(t_ah<S>() + o);
+      f.al[i] = *reinterpret_cast<const bf16x8*>(t_al<S>() + o);
+    }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int o = (wm * 128 + i * 32) * kRowB + rowoff + kc;
-        a_h[i] = *reinterpret_cast<const bf16x8*>(t_ah<S>() + o);
-        a_l[i] = *reinterpret_cast<const bf16x8*>(t_al<S>() + o);
-      }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int o = (wn * 64 + j * 32) * kRowB + rowoff + kc;
-        b_h[j] = *reinterpret_cast<const bf16x8*>(t_bh<S>() + o);
-        b_l[j] = *reinterpret_cast<const bf16x8*>(t_bl<S>() + o);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_l[i], b_h[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_l[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_h[i], b_h[j], acc[i][j], 0, 0, 0);
-        }
+    for (int j = 0; j < 4; ++j) {
+      const int o = (wn * 128 + j * 32 + lr) * kRowB + kc;
+      f.bh[j] = *reinterpret_cast<const bf16x8*>(t_bh<S>() + o);
+      f.bl[j] = *reinterpret_cast<const bf16x8*>(t_bl<S>() + o);
     }
   }
 
+  __device__ __forceinline__ static void mma(f32x16 (&acc)[4][4], const Frags& f) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+      }
+  }
+
+  // One stage (32 k) of the wave's tile: the fragments of the second 16-k step are read
+  // while the 48 MFMAs of the first run (one read after each of the first 16 MFMAs).
   template <int S>
-  __device__ __forceinline__ void step(int q, f32x16 (&acc)[4][2], int wm, int wn) const {
-    // the loads of stage q are the older kL of the 2*kL in flight
-    __builtin_amdgcn_s_waitcnt(0xF70 | kL);
+  __device__ __forceinline__ void compute(f32x16 (&acc)[4][4], int wm, int wn) const {
+    Frags f0, f1;
+    load<S>(f0, 0, wm, wn);
+    // all of f0 has landed here (one wait), and no read of f1 moves above this point: the
+    // MFMAs on f0 then need no waits, and only the f1 reads interleave with them
+    asm volatile(""
+                 : "+v"(f0.ah[0]), "+v"(f0.ah[1]), "+v"(f0.ah[2]), "+v"(f0.ah[3]),
+                   "+v"(f0.al[0]), "+v"(f0.al[1]), "+v"(f0.al[2]), "+v"(f0.al[3]),
+                   "+v"(f0.bh[0]), "+v"(f0.bh[1]), "+v"(f0.bh[2]), "+v"(f0.bh[3]),
+                   "+v"(f0.bl[0]), "+v"(f0.bl[1]), "+v"(f0.bl[2]), "+v"(f0.bl[3])
+                 :
+                 : "memory");
+    load<S>(f1, 1, wm, wn);
+    mma(acc, f0);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
+    mma(acc, f1);
+  }
+
+  template <int S>
+  __device__ __forceinline__ void step(int q, f32x16 (&acc)[4][4], int wm, int wn) const {
+    // vmcnt = 0: this wave's loads of stage q (the only ones in flight) have landed;
+    // lgkmcnt = 0: its reads of stage q-1 are done -- the loads issued after the barrier
+    // overwrite the buffer stage q-1 read
+    __builtin_amdgcn_s_waitcnt(0x070);
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    issue<(S + 2) % 3>(q + 2);
+    issue<(S + 1) % 2>(q + 1);
     if (q < nk) compute<S>(acc, wm, wn);
   }
 };
 
 // K is cut into slices of kst stages (blockIdx.y = slice); slice z accumulates its stages
-// in FP32 and writes its partial product to C + z * slice_stride.
+// in FP32 and writes its partial product to C + z * slice_stride.  256 threads = 2 x 2
+// waves of 128 x 128 (4 x 4 MFMA tiles of 32 x 32, accumulators in AGPRs).
 __global__ __launch_bounds__(256, 1) void k_gemm_split(
     const __bf16* __restrict__ ah, const __bf16* __restrict__ al,
     const __bf16* __restrict__ bh, const __bf16* __restrict__ bl, float* __restrict__ C,
@@ -167,28 +198,24 @@ __global__ __launch_bounds__(256, 1) void k_gemm_split(
     g.boff[u] = (n0 + row) * Kp + kcl * 8;
   }
   const int wm = g.wave >> 1, wn = g.wave & 1;
-  f32x16 acc[4][2];
+  f32x16 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
   g.issue<0>(0);
-  g.issue<1>(1);
-  const int ng = (g.nk + 2) / 3;
-  for (int grp = 0; grp < ng; ++grp) {
-    const int q = grp * 3;
+  for (int q = 0; q < g.nk; q += 2) {
     g.step<0>(q, acc, wm, wn);
     g.step<1>(q + 1, acc, wm, wn);
-    g.step<2>(q + 2, acc, wm, wn);
   }
   __builtin_amdgcn_s_waitcnt(0xF70);   // the trailing (clamped) loads
   // C[d, r]: lane -> d = lane%32 of the 32-wide tile, rows 8*(e/4) + 4*(lane/32) + e%4
   const int lr = g.lane & 31, hi = g.lane >> 5;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int64_t d = n0 + wn * 64 + j * 32 + lr;
+  for (int j = 0; j < 4; ++j) {
+    const int64_t d = n0 + wn * 128 + j * 32 + lr;
     if (d >= N) continue;
     float* crow = C + d * ldc + m0 + wm * 128;
 #pragma unroll
