@@ -113,19 +113,27 @@ struct SplitGemm {
     }
   }
 
+  // One of the three products (0: lo*hi, 1: hi*lo, 2: hi*hi, applied in this order) over
+  // the 16 tiles.
+  template <int PR>
   __device__ __forceinline__ static void mma(f32x16 (&acc)[4][4], const Frags& f) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+        if constexpr (PR == 0)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+        else if constexpr (PR == 1)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+        else
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
       }
   }
 
-  // One stage (32 k) of the wave's tile: the fragments of the second 16-k step are read
-  // while the 48 MFMAs of the first run (one read after each of the first 16 MFMAs).
+  // One stage (32 k) of the wave's tile.  The products run as three groups of 16 MFMAs
+  // kept apart by scheduling barriers, so that the three MFMAs on one accumulator are 16
+  // apart instead of back to back (each waiting for the previous result); the fragment
+  // reads of the second 16-k step are interleaved with the first group.
   template <int S>
   __device__ __forceinline__ void compute(f32x16 (&acc)[4][4], int wm, int wn) const {
     Frags f0, f1;
@@ -140,14 +148,22 @@ struct SplitGemm {
                  :
                  : "memory");
     load<S>(f1, 1, wm, wn);
-    mma(acc, f0);
+    mma<0>(acc, f0);
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
       __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
     }
-    __builtin_amdgcn_sched_group_barrier(0x008, 32, 0);
-    mma(acc, f1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma<1>(acc, f0);
+    __builtin_amdgcn_sched_barrier(0);
+    mma<2>(acc, f0);
+    __builtin_amdgcn_sched_barrier(0);
+    mma<0>(acc, f1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma<1>(acc, f1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma<2>(acc, f1);
   }
 
   template <int S>
