@@ -418,7 +418,7 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
 }
 
 // per-class segment counts for the merge (waves [gw_end[c-1], gw_end[c]) use nseg[c])
-constexpr int kMaxClass = 16;
+constexpr int kMaxClass = 32;
 struct SegMap {
   int n;
   int gw_end[kMaxClass];
@@ -892,9 +892,13 @@ const GatherCfg kGatherCfgs[] = {
     RSP_CFG(8, 4),  RSP_CFG(12, 4), RSP_CFG(16, 4), RSP_CFG(20, 4), RSP_CFG(24, 4),
     // frames split over X lanes
     RSP_CFGX(96, 1, 2), RSP_CFGX(96, 1, 4), RSP_CFGX(96, 1, 8),
+    RSP_CFGX(96, 1, 16), RSP_CFGX(96, 1, 32), RSP_CFGX(96, 1, 64),
     RSP_CFGX(40, 2, 2), RSP_CFGX(40, 2, 4), RSP_CFGX(40, 2, 8),
+    RSP_CFGX(40, 2, 16), RSP_CFGX(40, 2, 32), RSP_CFGX(40, 2, 64),
     RSP_CFGX(24, 3, 2), RSP_CFGX(24, 3, 4), RSP_CFGX(24, 3, 8),
+    RSP_CFGX(24, 3, 16), RSP_CFGX(24, 3, 32), RSP_CFGX(24, 3, 64),
     RSP_CFGX(16, 4, 2), RSP_CFGX(16, 4, 4), RSP_CFGX(16, 4, 8),
+    RSP_CFGX(16, 4, 16), RSP_CFGX(16, 4, 32), RSP_CFGX(16, 4, 64),
 };
 #undef RSP_CFG
 #undef RSP_CFGX
@@ -1082,7 +1086,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   // lanes per conflict group: 32 banks -> half waves, 64 -> full waves, 0 -> no scheduling
   const int G = NB == 0 ? 64 : NB;
   // Waves of 64 lanes over the sorted frames.  A frame whose entries do not fit the largest
-  // instantiation is split over X lanes (X = 2, 4, 8; entry r goes to lane r % X of its
+  // instantiation is split over X lanes (X = 2 .. 64; entry r goes to lane r % X of its
   // group), each wave then holding 64/X frames.  Per wave and part: slot -> LDS entry
   // (within the part) per lane, -1 = free slot.
   struct Wave {
@@ -1159,7 +1163,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
     for (;; X *= 2) {
       build_wave(f0, X, wv);
       if (wv.need <= max_slots(e->NP, X)) break;
-      if (X == 8)
+      if (X == 64)
         return fail(kErrUnsupported, "rsp_emc_set_frames: a frame has too many scored photons");
     }
     f0 += wv.nf;

@@ -182,3 +182,42 @@ def test_estep_multi_part_tables(size):
     bad = np.nonzero(emc.rmax != rmax)[0]
     assert np.all(margin[bad] < 1e-9 * np.abs(mx[bad])), (bad, margin[bad])
     np.testing.assert_allclose(emc.maxll, mx, rtol=1e-12, atol=1e-9)
+
+
+def test_estep_mixed_split_and_whole_frames():
+    """Frames heavy enough to be split over 2-64 lanes come first (their waves hold 32 .. 1
+    frames, which shifts the lane groups of the whole-frame waves after them), then
+    regrouped whole-frame waves: the argmax still matches the oracle, with the regrouping
+    on or off."""
+    import torch
+    from oracle import emc_oracle as O
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.emc import EMC
+    size = 141
+    cfg = synth.SynthConfig(size=size, num_data=300, mean_count=200., n_blobs=20,
+                            disk_radius=8., beamstop=4., shift_max=1., seed=11,
+                            fluence='lognormal', fluence_sigma=1.0)
+    fr = synth.make_frames(cfg)
+    n = fr.num_data
+    counts = fr.ones + np.bincount(np.repeat(np.arange(n), fr.multi), weights=fr.count_multi,
+                                   minlength=n)
+    # up to ~13x the median: frames split over 2 .. 64 lanes beside whole frames
+    assert counts.max() > 8 * np.median(counts)
+    grid = dict(shiftx=(-1., 1., 2), shifty=(0., 0., 1), sphere_dia=(7., 9., 2))
+    rng = np.random.default_rng(5)
+    model = 0.05 * (rng.random(size * size) + 1j * rng.random(size * size))
+    ofr = O.SparseFramesNP(fr.ones, fr.multi, fr.place_ones, fr.place_multi, fr.count_multi,
+                           fr.num_pix)
+    sx, sy, dia, ns = O.state_grid(grid['shiftx'], grid['shifty'], grid['sphere_dia'])
+    prob, msums, vscale = O.estep_prob(model, size, (sx, sy, dia), 5, ofr)
+    mx, rmax = O.get_rmax(prob)
+    margin = O.top_margin(prob)
+    for iters in (None, 0):
+        emc = EMC(dataset=Dataset(frames=fr), size=size, num_rot=5, model=model, save=False,
+                  regroup_iters=iters, **grid)
+        emc._calculate_prob(torch.as_tensor(model, device='cuda'))
+        emc._get_rmax()
+        bad = np.nonzero(emc.rmax != rmax)[0]
+        assert np.all(margin[bad] < 1e-9 * np.abs(mx[bad])), (iters, bad, margin[bad])
+        np.testing.assert_allclose(emc.maxll, mx, rtol=1e-12, atol=1e-9)
