@@ -174,7 +174,9 @@ int rsp_rot_set_frames(rsp_rot* plan, int64_t num_data, const int32_t* ones,
 /* Keys: 1 = cross-term precision (0 FP32 rocBLAS, 1 split BF16 MFMA = default, 2 FP64
  * with an FP64 log term), 2 = frames per GEMM chunk (0 = about 8 GB of operand rows;
  * 4096 for the adjoint), 3 = number of object classes C (default 1), 5 = per-kernel
- * timing (read with rsp_rot_timing).  Refused while a pass is open. */
+ * timing (read with rsp_rot_timing), 6 = precision of the frame-gradient GEMM
+ * Obar = P O of the adjoint (0 FP64 rocBLAS = default, 1 split BF16 MFMA: about 1e-5
+ * relative in Obar, for gradient refinement).  Refused while a pass is open. */
 int rsp_rot_set_option(rsp_rot* plan, int key, int64_t value);
 /* model_c: device c16 npix; sx, sy, dia, scales: device f64 N; ll, prob: device f64 N*R
  * (either may be NULL); logz: device f64 N (may be NULL).  Synchronises `stream`. */

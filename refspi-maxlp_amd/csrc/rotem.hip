@@ -580,16 +580,36 @@ __device__ __forceinline__ double block_sum256(double v, double* red) {
   return red[0] + red[1] + red[2] + red[3];
 }
 
+// A planes of the split-BF16 frame-gradient GEMM Obar = P A: row 2j (2j+1) holds Re (Im)
+// O_h(j) over h = 0 .. Hp-1 (zero for h >= H), from the pixel-major FP64 copy OT; each
+// value rounded to FP32, then split into bf16 hi + lo.  Rows >= 2K stay as allocated (zero).
+__global__ void k_ot_planes(const double2* __restrict__ OT, int K, int H, int Hs, int64_t Hp,
+                            __bf16* __restrict__ hi, __bf16* __restrict__ lo) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)K * Hp) return;
+  const int64_t j = t / Hp;
+  const int h = (int)(t - j * Hp);
+  const double2 o = h < H ? OT[j * Hs + h] : make_double2(0., 0.);
+  const float re = (float)o.x, im = (float)o.y;
+  const __bf16 rh = (__bf16)re, ih = (__bf16)im;
+  hi[(2 * j) * Hp + h] = rh;
+  lo[(2 * j) * Hp + h] = (__bf16)(re - (float)rh);
+  hi[(2 * j + 1) * Hp + h] = ih;
+  lo[(2 * j + 1) * Hp + h] = (__bf16)(im - (float)ih);
+}
+
 // One work-group per frame of [d0, d0 + gridDim.x): frame gradients (sx, sy, dia).
 // Dense part: threads over the scored pixels (Obar row dl of the sub-chunk).  Sparse part:
 // wave w takes photons w, w+4, ...; lanes over hypotheses h = lane + 64 t.
-template <int NT>
+// Obar rows: (re, im) pairs of TO (double: the FP64 GEMM; float: the split-BF16 one),
+// row stride ldo elements.
+template <int NT, typename TO>
 __global__ __launch_bounds__(256) void k_adj_frame(
     int size, int K, int H, int Hs, double rel, const int32_t* __restrict__ scored,
     const int32_t* __restrict__ jrad, const double* __restrict__ rad, int nrad,
     const int64_t* __restrict__ fo, const int32_t* __restrict__ fj,
     const float* __restrict__ fk, const double2* __restrict__ OT, const double* __restrict__ P,
-    const double* __restrict__ Obar, int64_t Kp, const double* __restrict__ sx,
+    const TO* __restrict__ Obar, int64_t ldo, const double* __restrict__ sx,
     const double* __restrict__ sy, const double* __restrict__ dia,
     const double* __restrict__ scales, int64_t d0, double* __restrict__ gsx,
     double* __restrict__ gsy, double* __restrict__ gdia) {
@@ -602,11 +622,11 @@ __global__ __launch_bounds__(256) void k_adj_frame(
   __syncthreads();
   // dense: sum_j Re(conj(Obar + R) dR)
   double ex_ = 0., ey_ = 0., ed_ = 0.;
-  const double2* ob = reinterpret_cast<const double2*>(Obar + dl * Kp);
+  const TO* ob = Obar + dl * ldo;
   for (int j = threadIdx.x; j < K; j += 256) {
     double2 Rv, Dx, Dy, Dd;
     ref_and_derivs(tb, scored[j], jrad[j], size, Rv, Dx, Dy, Dd);
-    const double2 o = ob[j];
+    const double2 o = make_double2((double)ob[2 * j], (double)ob[2 * j + 1]);
     const double zr = o.x + Rv.x, zi = o.y + Rv.y;
     ex_ += zr * Dx.x + zi * Dx.y;
     ey_ += zr * Dy.x + zi * Dy.y;
@@ -833,6 +853,11 @@ struct rsp_rot {
   double *d_aA = nullptr, *d_aB = nullptr, *d_aObar = nullptr, *d_aQ = nullptr;
   double *d_aGT = nullptr, *d_apart = nullptr, *d_anormO = nullptr, *d_anormR = nullptr;
   double2 *d_aOT = nullptr, *d_aGO = nullptr;
+  // frame gradients with a split-BF16 Obar (option 6): A planes [2 x fg_Mp x fg_Hp] of the
+  // slices (r = pixel component, k = hypothesis), B planes [2 x fg_Np x fg_Hp] of P
+  int fg_split = 0;
+  int64_t fg_Mp = 0, fg_Np = 0, fg_Hp = 0;
+  __bf16 *d_fgA = nullptr, *d_fgB = nullptr;
   // streamed pass
   int pass_flags = -1;
   int64_t pass_max = 0, pc_alloc = 0;
@@ -870,6 +895,8 @@ void free_adj(rsp_rot* p) {
   dfree(p->d_anormR);
   dfree(p->d_aOT);
   dfree(p->d_aGO);
+  dfree(p->d_fgA);
+  dfree(p->d_fgB);
   p->adj_chunk = 0;
 }
 
@@ -928,6 +955,16 @@ int ensure_adj(rsp_rot* p, int64_t chunk) {
     free_adj(p);
     return rc;
   }
+  // split-BF16 Obar operands (rows >= 2K of the A planes stay zero)
+  p->fg_Mp = xterm_pad_m(2 * (int64_t)p->K);
+  p->fg_Hp = xterm_pad_k(p->H);
+  p->fg_Np = xterm_pad_n(chunk);
+  if ((rc = dalloc(&p->d_fgA, 2 * p->fg_Mp * p->fg_Hp)) ||
+      (rc = dalloc(&p->d_fgB, 2 * p->fg_Np * p->fg_Hp))) {
+    free_adj(p);
+    return rc;
+  }
+  RSP_HIP(hipMemset(p->d_fgA, 0, sizeof(__bf16) * (size_t)(2 * p->fg_Mp * p->fg_Hp)));
   // operand padding (rows >= H of A, columns >= 2K of A and B) stays zero
   RSP_HIP(hipMemset(p->d_aA, 0, sizeof(double) * (size_t)(p->Mpad * p->Kp)));
   RSP_HIP(hipMemset(p->d_aB, 0, sizeof(double) * (size_t)(chunk * p->Kp)));
@@ -1115,10 +1152,17 @@ template <int NT>
 void launch_adj_frame_nt(rsp_rot* p, const Csr& f, const double* prob, const double* sx,
                          const double* sy, const double* dia, const double* scales, int64_t d0,
                          int64_t n, double* gsx, double* gsy, double* gdia, hipStream_t st) {
-  hipLaunchKernelGGL((k_adj_frame<NT>), dim3((unsigned)n), dim3(256), adj_frame_shmem(p), st,
-                     p->size, p->K, p->H, p->Hs, p->rel, p->d_scored, p->d_jrad, p->d_rad,
-                     p->nrad, f.fo, f.fj, f.fk, p->d_aOT, prob, p->d_aObar, p->Kp, sx, sy, dia,
-                     scales, d0, gsx, gsy, gdia);
+  if (p->fg_split)   // Obar in FP32 from the split GEMM, row stride fg_Mp
+    hipLaunchKernelGGL((k_adj_frame<NT, float>), dim3((unsigned)n), dim3(256),
+                       adj_frame_shmem(p), st, p->size, p->K, p->H, p->Hs, p->rel, p->d_scored,
+                       p->d_jrad, p->d_rad, p->nrad, f.fo, f.fj, f.fk, p->d_aOT, prob,
+                       reinterpret_cast<const float*>(p->d_aObar), p->fg_Mp, sx, sy, dia,
+                       scales, d0, gsx, gsy, gdia);
+  else
+    hipLaunchKernelGGL((k_adj_frame<NT, double>), dim3((unsigned)n), dim3(256),
+                       adj_frame_shmem(p), st, p->size, p->K, p->H, p->Hs, p->rel, p->d_scored,
+                       p->d_jrad, p->d_rad, p->nrad, f.fo, f.fj, f.fk, p->d_aOT, prob,
+                       p->d_aObar, p->Kp, sx, sy, dia, scales, d0, gsx, gsy, gdia);
 }
 
 int launch_adj_frame(rsp_rot* p, const Csr& f, const double* prob, const double* sx,
@@ -1146,6 +1190,13 @@ int run_adj_frames(rsp_rot* p, const Csr& f, const double* prob, const double* s
                    hipStream_t st) {
   const int64_t chunk = p->adj_chunk;
   int rc;
+  if (want_frame && p->fg_split) {
+    const int64_t nt = (int64_t)p->K * p->fg_Hp;
+    hipLaunchKernelGGL(k_ot_planes, dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0, st,
+                       p->d_aOT, p->K, p->H, p->Hs, p->fg_Hp, p->d_fgA,
+                       p->d_fgA + p->fg_Mp * p->fg_Hp);
+    RSP_LAUNCH_CHECK("k_ot_planes");
+  }
   for (int64_t d0 = 0; d0 < f.n; d0 += chunk) {
     const int64_t n = std::min(chunk, f.n - d0);
     if ((rc = launch_ref_to<kFP64>(p, sx, sy, dia, d0, n, f.n, p->d_aB, chunk, p->d_anormR, st)))
@@ -1160,10 +1211,21 @@ int run_adj_frames(rsp_rot* p, const Csr& f, const double* prob, const double* s
         return rc;
     }
     if (want_frame) {
-      // Obar (n x Kp, row-major) = P A : Obar[d * Kp + k] = sum_h P[d, h] A[h, k]
-      if ((rc = xterm_dgemm(false, false, p->Kp, n, p->H, 1., p->d_aA, p->Kp, prob + d0 * p->H,
-                            p->H, 0., p->d_aObar, p->Kp, st)))
+      if (p->fg_split) {
+        // Obar (n x fg_Mp FP32, row-major) = P A on MFMA: r = k (2K pixel components),
+        // d = frame, reduction over h (one K slice: H <= 1536 < 12288)
+        if ((rc = xterm_split64(prob + d0 * p->H, n, p->H, p->H, p->d_fgB, p->d_fgB + p->fg_Np * p->fg_Hp,
+                                p->fg_Np, p->fg_Hp, st)) ||
+            (rc = xterm_gemm_split(p->d_fgA, p->d_fgA + p->fg_Mp * p->fg_Hp, p->d_fgB,
+                                   p->d_fgB + p->fg_Np * p->fg_Hp,
+                                   reinterpret_cast<float*>(p->d_aObar), p->fg_Mp, n, p->fg_Hp,
+                                   p->fg_Mp, 0, st)))
+          return rc;
+      } else if ((rc = xterm_dgemm(false, false, p->Kp, n, p->H, 1., p->d_aA, p->Kp,
+                                   prob + d0 * p->H, p->H, 0., p->d_aObar, p->Kp, st))) {
+        // Obar (n x Kp, row-major) = P A : Obar[d * Kp + k] = sum_h P[d, h] A[h, k]
         return rc;
+      }
       if ((rc = launch_adj_frame(p, f, prob, sx, sy, dia, scales, d0, n, gsx, gsy, gdia, st)))
         return rc;
     }
@@ -1430,6 +1492,10 @@ int rsp_rot_set_option(rsp_rot* p, int key, int64_t value) {
     }
     case 5:
       p->timing = value != 0;
+      return kOk;
+    case 6:
+      if (value < 0 || value > 1) return fail(kErrSize, "frame-gradient GEMM must be 0 or 1");
+      p->fg_split = (int)value;
       return kOk;
     default:
       return fail(kErrUnsupported, "unknown option key");

@@ -266,6 +266,26 @@ __global__ void k_split_rows(const float* __restrict__ X, int64_t rows, int64_t 
   *reinterpret_cast<bf16x8*>(lo + r * Kp + k0) = l;
 }
 
+// As k_split_rows for FP64 rows (each value rounded to FP32 first).
+__global__ void k_split_rows64(const double* __restrict__ X, int64_t rows, int64_t K,
+                               int64_t ld, __bf16* __restrict__ hi, __bf16* __restrict__ lo,
+                               int64_t rows_pad, int64_t Kp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per_row = Kp / 8;
+  if (t >= rows_pad * per_row) return;
+  const int64_t r = t / per_row, k0 = (t - r * per_row) * 8;
+  bf16x8 h, l;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float x = (r < rows && k0 + e < K) ? (float)X[r * ld + k0 + e] : 0.f;
+    const __bf16 xh = (__bf16)x;
+    h[e] = xh;
+    l[e] = (__bf16)(x - (float)xh);
+  }
+  *reinterpret_cast<bf16x8*>(hi + r * Kp + k0) = h;
+  *reinterpret_cast<bf16x8*>(lo + r * Kp + k0) = l;
+}
+
 // C[d * M + r] = sum over the S slices (FP64, fixed order) of P[z * stride + d * ldp + r].
 __global__ void k_slice_sum(const float* __restrict__ P, int64_t stride, int S, int64_t M,
                             int64_t ldp, int64_t n, float* __restrict__ C) {
@@ -311,6 +331,16 @@ int xterm_split(const float* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi,
   hipLaunchKernelGGL(k_split_rows, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, X,
                      rows, K, ld, hi, lo, rows_pad, Kp);
   RSP_LAUNCH_CHECK("k_split_rows");
+  return kOk;
+}
+
+int xterm_split64(const double* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi,
+                  __bf16* lo, int64_t rows_pad, int64_t Kp, hipStream_t stream) {
+  const int64_t n = rows_pad * (Kp / 8);
+  if (n == 0) return kOk;
+  hipLaunchKernelGGL(k_split_rows64, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, X,
+                     rows, K, ld, hi, lo, rows_pad, Kp);
+  RSP_LAUNCH_CHECK("k_split_rows64");
   return kOk;
 }
 

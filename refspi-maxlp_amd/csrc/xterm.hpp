@@ -7,7 +7,7 @@
 
 namespace rsp {
 
-// Padding the split kernel needs: rows of A to 256, rows of B to 128, K to 32.
+// Padding the split kernel needs: rows of A and of B to 256, K to 32.
 int64_t xterm_pad_m(int64_t M);
 int64_t xterm_pad_n(int64_t N);
 int64_t xterm_pad_k(int64_t K);
@@ -15,6 +15,10 @@ int64_t xterm_pad_k(int64_t K);
 // FP32 rows -> bf16 hi/lo planes (zero padded to rows_pad x Kp).
 int xterm_split(const float* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi, __bf16* lo,
                 int64_t rows_pad, int64_t Kp, hipStream_t stream);
+
+// FP64 rows -> bf16 hi/lo planes (each value rounded to FP32 first; zero padded).
+int xterm_split64(const double* X, int64_t rows, int64_t K, int64_t ld, __bf16* hi,
+                  __bf16* lo, int64_t rows_pad, int64_t Kp, hipStream_t stream);
 
 // Number of K slices of the split GEMM for padded K = Kp (each slice accumulates 12288 K
 // elements in FP32).

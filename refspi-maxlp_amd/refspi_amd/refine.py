@@ -36,15 +36,30 @@ def _pass(plan, models, sx, sy, dia, scales, grad):
 
 
 def refine(plan, models, sx, sy, dia, scales=None, steps=4, radius=0.25, grow=1.5,
-           shrink=0.3, min_radius=1e-3, on_step=None):
+           shrink=0.3, min_radius=1e-3, on_step=None, grad_precision=None):
     """Trust-region ascent of every frame's log Z_d over (sx, sy, dia).
 
     Each step moves a frame by ``radius_d`` (px) along its gradient of log Z_d (the three
     parameters in pixels, unscaled), evaluates log Z_d there, and keeps the move only if
     log Z_d rose; the frame's radius then grows by ``grow``, else shrinks by ``shrink``.
-    One pass (E-step + frame adjoint) per step.  Returns (sx, sy, dia, logz, history) with
+    One pass (E-step + frame adjoint) per step.  ``grad_precision``: the frame-gradient
+    GEMM's ('fp64' or 'split'; default 'split' for a split-mode plan, else 'fp64') -- only
+    the direction of a step depends on it.  Returns (sx, sy, dia, logz, history) with
     device tensors; history[i] = (mean log Z, accepted fraction).
     """
+    if grad_precision is None:
+        grad_precision = 'split' if plan.mode == 'split' else 'fp64'
+    before = plan.frame_grad
+    plan.set_frame_grad(grad_precision)
+    try:
+        return _refine(plan, models, sx, sy, dia, scales, steps, radius, grow, shrink,
+                       min_radius, on_step)
+    finally:
+        plan.set_frame_grad(before)
+
+
+def _refine(plan, models, sx, sy, dia, scales, steps, radius, grow, shrink, min_radius,
+            on_step):
     import torch
     f64 = torch.float64
     dev = 'cuda:%d' % plan.device

@@ -62,6 +62,7 @@ class _Plan:
         if self.classes != 1:
             _lib.call('rsp_rot_set_option', h, 3, self.classes)
         self.num_hyp = self.classes * num_rot
+        self.frame_grad = 'fp64'
         self.set_mode(mode)
         if chunk:
             _lib.call('rsp_rot_set_option', h, 2, int(chunk))
@@ -72,6 +73,15 @@ class _Plan:
 
     def set_timing(self, on=True):
         _lib.call('rsp_rot_set_option', self._h, 5, 1 if on else 0)
+
+    def set_frame_grad(self, precision):
+        """Precision of the frame-gradient GEMM Obar = P O of the adjoint: 'fp64'
+        (rocBLAS, the default) or 'split' (BF16 hi/lo on MFMA, about 1e-5 relative in Obar;
+        for gradient refinement, where a step is kept only if log Z rose)."""
+        if precision not in ('fp64', 'split'):
+            raise ValueError("frame-gradient precision must be 'fp64' or 'split'")
+        _lib.call('rsp_rot_set_option', self._h, 6, 1 if precision == 'split' else 0)
+        self.frame_grad = precision
 
     def __del__(self):
         h = getattr(self, '_h', None)
