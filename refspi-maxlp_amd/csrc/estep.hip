@@ -146,6 +146,58 @@ __device__ __forceinline__ double logview_exact(const double* __restrict__ Is,
 }
 
 // --------------------------------------------------------------------------------
+// The FP64 natural log of the tables: Cody-Waite argument reduction v = 2^e m with
+// m in [sqrt(1/2), sqrt(2)) and ln 2 split into hi (32 significant bits: e * hi is exact)
+// + lo, and log m = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716, by its series up to
+// s^15 (truncation < 1.3e-14).  Measured against an 80-bit log (with a reciprocal start
+// 2^-23 off): |error| <= 6.9e-14 for v in [1e-20, 1e300] (1.2e-14 for v in [0.5, 2)).  About a quarter of the instructions of
+// the library log, which the exact path keeps; kTabLogErr, added to every table entry's
+// error bound, covers the difference of the two (by four orders of magnitude).
+// --------------------------------------------------------------------------------
+constexpr double kTabLogErr = 1e-9;
+
+__device__ __forceinline__ double log_table(double v) {
+  int e;
+  double m = frexp(v, &e);  // m in [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m *= 2.;
+    --e;
+  }
+  // s = (m - 1)/(m + 1) through the hardware reciprocal and two Newton steps (a start
+  // 2^-23 off ends within FP64 rounding)
+  const double dn = m + 1.;
+  double rc = __builtin_amdgcn_rcp(dn);
+  rc = fma(rc, fma(-dn, rc, 1.), rc);
+  rc = fma(rc, fma(-dn, rc, 1.), rc);
+  const double s = (m - 1.) * rc;
+  const double z = s * s;
+  double p = 1. / 15.;
+  p = fma(p, z, 1. / 13.);
+  p = fma(p, z, 1. / 11.);
+  p = fma(p, z, 1. / 9.);
+  p = fma(p, z, 1. / 7.);
+  p = fma(p, z, 1. / 5.);
+  p = fma(p, z, 1. / 3.);
+  p = fma(p, z, 1.);
+  const double ln2_hi = 0x1.62e42feep-1, ln2_lo = 0x1.a39ef35793c76p-33;
+  return fma((double)e, ln2_hi, fma((double)e, ln2_lo, 2. * s * p));
+}
+
+// logview_exact with log_table: the same sample v (bit for bit), its log within
+// kTabLogErr of the exact path's
+__device__ __forceinline__ double logview_table(const double* __restrict__ Is,
+                                                const double* __restrict__ bg, int size,
+                                                int t, double ac, double as) {
+  const int x = t / size, y = t - x * size;
+  double tx, ty, v;
+  rot_source(x, y, size / 2, ac, as, tx, ty);
+  if (!bilinear(Is, size, tx, ty, v)) return -1000.;
+  v = v * 1.;
+  v = v + bg[t];
+  return v < 1.e-20 ? -1000. : log_table(v);
+}
+
+// --------------------------------------------------------------------------------
 // 3a: FP32 tables for hypotheses [hb, hb+nb)
 // tables[(h-hb)*TS + part*PSTRIDE + (i - part*kPartMax)] = fl32(logview)
 // --------------------------------------------------------------------------------
@@ -166,11 +218,12 @@ __global__ __launch_bounds__(256) void k_tables(
   float emax = 0.f, amax = 0.f;
   for (int h = h0; h < h1; ++h) {
     const int s = h / R, r = h - s * R;
-    const double lv = logview_exact(I + (int64_t)s * npix, bg, size, t, rcos[r], rsin[r]);
+    const double lv = logview_table(I + (int64_t)s * npix, bg, size, t, rcos[r], rsin[r]);
     const float f = (float)lv;
     tables[(int64_t)(h - hb) * TS + col] = f;
-    // the rounding error, rounded up to float
-    const double d = fabs((double)f - lv);
+    // the error against the exact path's log-view, rounded up to float: the FP32 rounding
+    // plus the two logs' difference
+    const double d = fabs((double)f - lv) + kTabLogErr;
     float df = (float)d;
     if ((double)df < d) df = nextafterf(df, INFINITY);
     emax = fmaxf(emax, df);
