@@ -173,6 +173,22 @@ __global__ __launch_bounds__(256) void k_moments(Geo g, const int32_t* __restric
   }
 }
 
+// Dispatch order of a k_pattern launch: key = (2^31 - 1 - entries of pixel t) << 32 | i for
+// list position i, so that an ascending sort puts the pixels with the most entries first
+// (ties in list order).  A pixel's work is proportional to its entries, which range over
+// two orders of magnitude (bright pixels near the centre); started last, the heaviest
+// would set the launch's tail.
+__global__ __launch_bounds__(256) void k_order_keys(const int32_t* __restrict__ pixels, int n,
+                                                    const int64_t* __restrict__ seg_beg,
+                                                    const int64_t* __restrict__ seg_end,
+                                                    uint64_t* __restrict__ keys) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int t = pixels[i];
+  const int64_t len = min(seg_end[t] - seg_beg[t], (int64_t)0x7fffffff);
+  keys[i] = ((uint64_t)(0x7fffffff - len) << 32) | (uint32_t)i;
+}
+
 __device__ __forceinline__ bool cand_better(double a, int ia, double b, int ib) {
   const bool na = isnan(a), nb = isnan(b);
   if (na || nb) return na && (!nb || ia < ib);
@@ -180,9 +196,12 @@ __device__ __forceinline__ bool cand_better(double a, int ia, double b, int ib) 
 }
 
 // One workgroup per pixel: up to 10 pattern-search steps (maxLP.py:154-179).
-// rescale < 0 selects the per-candidate rescale of phase 1 (maxLP.py:187).
+// rescale < 0 selects the per-candidate rescale of phase 1 (maxLP.py:187).  Work-group b
+// takes list position i = low 32 bits of order[b] (k_order_keys, sorted); the pixel's
+// model value and resc_out[i] are its only outputs, so the order changes no result.
 __global__ __launch_bounds__(kThreads) void k_pattern(
-    const int32_t* __restrict__ pixels, const int64_t* __restrict__ seg_beg,
+    const uint64_t* __restrict__ order, const int32_t* __restrict__ pixels,
+    const int64_t* __restrict__ seg_beg,
     const int64_t* __restrict__ seg_end, const Entry* __restrict__ ent,
     const double* __restrict__ mom, int N, double rescale, double* __restrict__ model,
     double* __restrict__ resc_out) {
@@ -190,7 +209,8 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
   __shared__ double wv[kThreads / 64];
   __shared__ int wi[kThreads / 64];
   __shared__ double skt;
-  const int t = pixels[blockIdx.x];
+  const int li = (int)(uint32_t)order[blockIdx.x];
+  const int t = pixels[li];
   const int64_t e0 = seg_beg[t], e1 = seg_end[t];
   const double m1r = mom[4 * t + 0], m1i = mom[4 * t + 1], m2 = mom[4 * t + 2];
   // K-bar = mean_d K_d (exact integer sum)
@@ -310,7 +330,7 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
     if (resc_out) {
       // maxLP.py:216-228 get_rescale_pixel at the final value
       const double mc = fr * fr + fi * fi + 2. * (fr * m1r + fi * m1i) + m2;
-      resc_out[blockIdx.x] = kbar / mc;
+      resc_out[li] = kbar / mc;
     }
   }
 }
@@ -328,7 +348,7 @@ struct rsp_maxlp {
   int32_t *d_mask = nullptr, *d_rmask_pix = nullptr, *d_mask_pix = nullptr, *d_fr = nullptr,
           *d_pix = nullptr, *d_cnt = nullptr, *d_nmatch = nullptr;
   int64_t *d_pos = nullptr, *d_seg_beg = nullptr, *d_seg_end = nullptr;
-  uint64_t *d_keys = nullptr, *d_keys2 = nullptr;
+  uint64_t *d_keys = nullptr, *d_keys2 = nullptr, *d_ord = nullptr, *d_ord2 = nullptr;
   Entry* d_ent = nullptr;
   int64_t key_cap = 0;
   void* d_tmp = nullptr;
@@ -401,7 +421,9 @@ extern "C" int rsp_maxlp_create(int device, int64_t size, rsp_maxlp** out) {
       (rc = upload(&m->d_rmask_pix, m->rmask_pix)) || (rc = upload(&m->d_mask_pix, m->mask_pix)) ||
       (rc = alloc(&m->d_mom, 4LL * m->npix)) || (rc = alloc(&m->d_seg_beg, m->npix)) ||
       (rc = alloc(&m->d_seg_end, m->npix)) ||
-      (rc = alloc(&m->d_resc, std::max<int64_t>(1, (int64_t)m->rmask_pix.size())))) {
+      (rc = alloc(&m->d_resc, std::max<int64_t>(1, (int64_t)m->rmask_pix.size()))) ||
+      (rc = alloc(&m->d_ord, (int64_t)std::max(m->rmask_pix.size(), m->mask_pix.size()))) ||
+      (rc = alloc(&m->d_ord2, (int64_t)std::max(m->rmask_pix.size(), m->mask_pix.size())))) {
     delete m;
     return rc;
   }
@@ -511,10 +533,24 @@ extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, co
                        sy, dia, m->d_mom);
     RSP_LAUNCH_CHECK("k_moments");
   }
+  // the pixel lists of a phase, heaviest first: d_ord2 (sorted keys of k_order_keys)
+  auto order = [&](const int32_t* pixels, int n) -> int {
+    hipLaunchKernelGGL(k_order_keys, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, pixels,
+                       n, m->d_seg_beg, m->d_seg_end, m->d_ord);
+    RSP_LAUNCH_CHECK("k_order_keys");
+    size_t tb = 0;
+    RSP_HIP(rocprim::radix_sort_keys(nullptr, tb, m->d_ord, m->d_ord2, (size_t)n, 0, 64, st));
+    int rc2 = ensure_tmp(m, tb);
+    if (rc2) return rc2;
+    RSP_HIP(rocprim::radix_sort_keys(m->d_tmp, tb, m->d_ord, m->d_ord2, (size_t)n, 0, 64, st));
+    return kOk;
+  };
   // phase 1: rmask annulus with per-candidate rescale (maxLP.py:317-320)
   if (nr > 0) {
-    hipLaunchKernelGGL(k_pattern, dim3(nr), dim3(kThreads), 0, st, m->d_rmask_pix, m->d_seg_beg,
-                       m->d_seg_end, m->d_ent, m->d_mom, (int)m->N, -1., model_c, m->d_resc);
+    if ((rc = order(m->d_rmask_pix, nr))) return rc;
+    hipLaunchKernelGGL(k_pattern, dim3(nr), dim3(kThreads), 0, st, (const uint64_t*)m->d_ord2,
+                       m->d_rmask_pix, m->d_seg_beg, m->d_seg_end, m->d_ent, m->d_mom,
+                       (int)m->N, -1., model_c, m->d_resc);
     RSP_LAUNCH_CHECK("k_pattern(phase1)");
   }
   // phase 2: mean rescale over the annulus (maxLP.py:324-329)
@@ -556,9 +592,10 @@ extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, co
   if (rescale_out) *rescale_out = rescale;
   // phase 3: full mask with the global rescale (maxLP.py:332-336)
   if (nmask > 0) {
-    hipLaunchKernelGGL(k_pattern, dim3(nmask), dim3(kThreads), 0, st, m->d_mask_pix,
-                       m->d_seg_beg, m->d_seg_end, m->d_ent, m->d_mom, (int)m->N, rescale,
-                       model_c, (double*)nullptr);
+    if ((rc = order(m->d_mask_pix, nmask))) return rc;
+    hipLaunchKernelGGL(k_pattern, dim3(nmask), dim3(kThreads), 0, st, (const uint64_t*)m->d_ord2,
+                       m->d_mask_pix, m->d_seg_beg, m->d_seg_end, m->d_ent, m->d_mom,
+                       (int)m->N, rescale, model_c, (double*)nullptr);
     RSP_LAUNCH_CHECK("k_pattern(phase3)");
   }
   RSP_HIP(hipStreamSynchronize(st));
@@ -580,7 +617,8 @@ extern "C" int rsp_maxlp_destroy(rsp_maxlp* m) {
   hipSetDevice(m->device);
   void* ptrs[] = {m->d_qx, m->d_qy, m->d_qrad, m->d_mom, m->d_resc, m->d_mask, m->d_rmask_pix,
                   m->d_mask_pix, m->d_fr, m->d_pix, m->d_cnt, m->d_nmatch, m->d_pos,
-                  m->d_seg_beg, m->d_seg_end, m->d_keys, m->d_keys2, m->d_ent, m->d_tmp};
+                  m->d_seg_beg, m->d_seg_end, m->d_keys, m->d_keys2, m->d_ord, m->d_ord2,
+                  m->d_ent, m->d_tmp};
   for (void* p : ptrs)
     if (p) hipFree(p);
   delete m;
