@@ -342,6 +342,9 @@ struct GatherStep {
   const float* tables;
   int TS, hb, h0, h1, nq, R, wid, lane;
   double scale;
+  // this lane's byte offset within each 1 KB chunk for parts 0 and 1 (the tile's table
+  // layout: 16 * lane for the identity)
+  unsigned off0, off1;
 
   // Loads of step q into dst: chunks 0 .. kChunks-2 as 16-byte-per-lane loads, L - 1 per
   // wave (chunk c from wave c % NW), and the last chunk as 4-byte-per-lane loads, one per
@@ -350,7 +353,10 @@ struct GatherStep {
   // path -- steps past the end re-load the first part (in bounds) into a buffer that is
   // never read for a real hypothesis -- so the wait counts are the same everywhere and
   // the compiler adds no waits.
+  // PP: the part of step q (q % NP) as a constant, which selects the lane's offset; a step
+  // past the end loads part 0 with it, in bounds, into a buffer no real step reads.
   static_assert((L - 1) * NW == kChunks - 1, "chunks split as (L - 1) per wave + 1");
+  template <int PP>
   __device__ __forceinline__ void issue(int q, float* dst) const {
     if (q >= nq) q = 0;
     const int h = h0 + q / NP, p = q - (q / NP) * NP;
@@ -358,7 +364,8 @@ struct GatherStep {
     // addresses as a wave-uniform base + one 32-bit lane offset (scalar-base loads: no
     // per-load address registers)
     const char* sb = reinterpret_cast<const char*>(src);
-    const unsigned l16 = (unsigned)lane * 16u, l4 = (unsigned)lane * 4u;
+    const unsigned l16 = NP > 2 ? (unsigned)lane * 16u : (PP == 0 ? off0 : off1),
+                   l4 = (unsigned)lane * 4u;
 #pragma unroll
     for (int u = 0; u < L - 1; ++u) {
       const int c = wid + u * NW;
@@ -392,7 +399,7 @@ struct GatherStep {
     // a bare barrier: a fenced __syncthreads would also wait for the loads of step q+1
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    issue(q + 2, part_buf<(QS + 2) % 3>());
+    issue<(QS + 2) % NP>(q + 2, part_buf<(QS + 2) % 3>());
     if constexpr (PP == 0) {
       acc[0] = init[min(h, h1 - 1) / R] * scale;
       acc[1] = acc[2] = acc[3] = 0.;
@@ -431,7 +438,8 @@ template <int S, int NP, int NW, int L, int X>
 __global__ __launch_bounds__(64 * NW) void k_gather(
     const float* __restrict__ tables, int TS, int hb, int nb, int nseg, int n_tiles, int gw0,
     const uint32_t* __restrict__ slots, const double* __restrict__ scale_sorted,
-    const double* __restrict__ init, int R, Top3* __restrict__ partials, int Npad) {
+    const double* __restrict__ init, int R, Top3* __restrict__ partials, int Npad,
+    const uint32_t* __restrict__ lay) {
   static_assert(L * NW >= kChunks, "L loads per wave must cover the part");
   constexpr int P = S * NP;
   const int b = blockIdx.x;
@@ -458,10 +466,13 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   Top3 top = top3_empty();
   // every wave issues exactly L direct-to-LDS loads per step; two steps are in flight
   // ahead of the gather
+  // lay[p * Npad + j]: the lane's chunk offset for part p (parts 0 and 1; more parts keep
+  // the identity)
   const GatherStep<S, NP, NW, L, X> st{tables, TS, hb, h0, h1, (h1 - h0) * NP, R, wid, lane,
-                                    scale_sorted[j]};
-  st.issue(0, g_buf0);
-  st.issue(1, g_buf1);
+                                    scale_sorted[j], NP <= 2 ? lay[j] : 0u,
+                                    NP == 2 ? lay[Npad + j] : 0u};
+  st.template issue<0>(0, g_buf0);
+  st.template issue<1 % NP>(1, g_buf1);
   double acc[4];
   const int ngroups = (h1 - h0 + 2) / 3;
   for (int g = 0; g < ngroups; ++g) st.template group<0>(g, a, init, acc, top);
@@ -703,6 +714,7 @@ struct rsp_emc {
   int P = 0, NW = 0, n_waves = 0, n_tiles = 0;
   double powder_sel_sum = 0.;
   uint32_t* d_slots = nullptr;
+  uint32_t* d_lay = nullptr;  // [part 0, 1][Npad]: per-lane chunk offsets of the table loads
   int32_t *d_meta = nullptr, *d_perm = nullptr;
   double* d_scale_sorted = nullptr;
   Top3 *d_partials = nullptr, *d_running = nullptr;
@@ -718,6 +730,7 @@ struct rsp_emc {
   int exact_mode = 0;
   int64_t opt_batch = 0, opt_nseg = 0;
   int regroup_iters = 4000;  // local-search swaps per window of 8 lane groups (0: off)
+  int layout = 1;            // per-tile LDS layout of the table parts (0: identity)
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   struct Class {
     int cfg, n_tiles, gw0;
@@ -756,7 +769,8 @@ int dev_upload(T** p, const std::vector<T>& v) {
 void free_all(rsp_emc* e) {
   void* ptrs[] = {e->d_rcos, e->d_rsin, e->d_sx, e->d_sy, e->d_dia, e->d_bg, e->d_I,
                   e->d_init, e->d_vscale, e->d_probmask, e->d_scored_t, e->d_compact_of,
-                  e->d_tables, e->d_errmax, e->d_absmax, e->d_slots, e->d_meta, e->d_perm,
+                  e->d_tables, e->d_errmax, e->d_absmax, e->d_slots, e->d_lay, e->d_meta,
+                  e->d_perm,
                   e->d_scale_sorted, e->d_partials, e->d_running, e->d_ex_off, e->d_ex_n1,
                   e->d_ex_pix, e->d_ex_cnt, e->d_full_list, e->d_counters, e->d_bval,
                   e->d_bidx};
@@ -766,16 +780,16 @@ void free_all(rsp_emc* e) {
   e->ev.clear();
 }
 
-// Greedy bank-aware slot assignment for one group of lanes of one wave and one table
-// part.  entries[l] lists LDS entry indices (within the part) for lane l; the result
-// places every entry of lane l in a distinct slot so that, within the group, no two
-// lanes read the same bank in the same slot.  Returns the number of slots used.
+// Bank-aware slot assignment for one group of lanes of one wave and one table part
+// (schedule_group).  entries[l] lists LDS entry indices (within the part) for lane l; the
+// result places every entry of lane l in a distinct slot so that, within the group, no two
+// lanes read the same bank in the same slot.  Returns the number of slots used: the larger
+// of the longest lane and the busiest bank, which no assignment can beat.
 // Regrouping of frames into lane groups before the waves are built.  The slot count of a
-// group of G lanes (a half-wave for 32 banks) on one part is at least
+// group of G lanes (a half-wave for 32 banks) on one part is
 // delta = max(largest per-part entry count of its frames, busiest bank of its entries),
-// and the greedy schedule of schedule_group stays within a fraction of a slot of it.  With
-// frames in descending order of entry count, the lane term is nearly flat over a window of
-// similar frames, but the bank term is a sum of G frames' bank histograms and sits well
+// which schedule_group reaches.  With frames in descending order of their largest per-part
+// entry count, the lane term is flat over a window of similar frames, but the bank term is a sum of G frames' bank histograms and sits well
 // above its mean.  Within a window of ng groups this randomised local search swaps frames
 // between groups -- always out of the group with the largest delta, the frame that loads
 // its busiest (part, bank) cell the most -- and keeps a swap when the larger of the two
@@ -867,46 +881,195 @@ void regroup_window(std::vector<int32_t>& order, int64_t f0, int64_t f1, int G,
 int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
                    std::vector<std::vector<int>>& slot_of) {
   const int G = (int)entries.size();
-  std::vector<uint64_t> bank_used;  // per slot, bit per bank
-  std::vector<std::vector<uint64_t>> lane_used(G);
   slot_of.assign(G, {});
   int max_len = 0;
   for (int l = 0; l < G; ++l) {
     slot_of[l].assign(entries[l].size(), -1);
     max_len = std::max(max_len, (int)entries[l].size());
   }
-  int nslots = 0;
   if (nbanks == 0) {  // plain packing: entry r of every lane in slot r
     for (int l = 0; l < G; ++l)
       for (size_t r = 0; r < entries[l].size(); ++r) slot_of[l][r] = (int)r;
     return max_len;
   }
-  auto lane_bit = [&](int l, int k) -> bool {
-    const auto& w = lane_used[l];
-    return (size_t)(k >> 6) < w.size() && ((w[k >> 6] >> (k & 63)) & 1ull);
-  };
-  for (int r = 0; r < max_len; ++r) {
-    for (int l = 0; l < G; ++l) {
-      if (r >= (int)entries[l].size()) continue;
-      const int bank = entries[l][r] % nbanks;
-      int k = 0;
-      for (;; ++k) {
-        if ((size_t)k >= bank_used.size()) bank_used.resize(k + 1, 0ull);
-        if (!((bank_used[k] >> bank) & 1ull) && !lane_bit(l, k)) break;
+  // An edge colouring of the bipartite multigraph lanes x banks (one edge per entry) with
+  // D = max degree colours (Koenig's theorem), colours being slots: entry (l, r) goes to
+  // the lowest slot a free at lane l when a is also free at its bank b; else to the lowest
+  // slot c free at b when c is free at l; else the path from b along edges of slots a, c
+  // alternately has its two slots exchanged -- it cannot reach l, which has no a-edge, and
+  // leaves a free at b -- and the entry takes a.
+  std::vector<int> bdeg(nbanks, 0);
+  for (int l = 0; l < G; ++l)
+    for (int v : entries[l]) ++bdeg[v % nbanks];
+  const int D = std::max(max_len, *std::max_element(bdeg.begin(), bdeg.end()));
+  if (D == 0) return 0;
+  std::vector<int> lane_at((size_t)G * D, -1);       // entry index r at (lane, slot)
+  std::vector<int> bank_at((size_t)nbanks * D, -1);  // (l << 16 | r) at (bank, slot)
+  std::vector<std::pair<int, int>> path;
+  for (int l = 0; l < G; ++l)
+    for (int r = 0; r < (int)entries[l].size(); ++r) {
+      const int b = entries[l][r] % nbanks;
+      int a = 0, c = 0;
+      while (lane_at[(size_t)l * D + a] >= 0) ++a;
+      while (bank_at[(size_t)b * D + c] >= 0) ++c;
+      int k;
+      if (bank_at[(size_t)b * D + a] < 0) {
+        k = a;
+      } else if (lane_at[(size_t)l * D + c] < 0) {
+        k = c;
+      } else {
+        path.clear();
+        int node = b, col = a;
+        bool at_bank = true;
+        for (;;) {
+          if (at_bank) {
+            const int id = bank_at[(size_t)node * D + col];
+            if (id < 0) break;
+            path.emplace_back(id >> 16, id & 0xffff);
+            node = id >> 16;
+          } else {
+            const int r2 = lane_at[(size_t)node * D + col];
+            if (r2 < 0) break;
+            path.emplace_back(node, r2);
+            node = entries[node][r2] % nbanks;
+          }
+          at_bank = !at_bank;
+          col = col == a ? c : a;
+        }
+        for (const auto& e : path) {
+          const int k0 = slot_of[e.first][e.second];
+          lane_at[(size_t)e.first * D + k0] = -1;
+          bank_at[(size_t)(entries[e.first][e.second] % nbanks) * D + k0] = -1;
+        }
+        for (const auto& e : path) {
+          const int k1 = slot_of[e.first][e.second] == a ? c : a;
+          slot_of[e.first][e.second] = k1;
+          lane_at[(size_t)e.first * D + k1] = e.second;
+          bank_at[(size_t)(entries[e.first][e.second] % nbanks) * D + k1] =
+              (e.first << 16) | e.second;
+        }
+        k = a;
       }
-      bank_used[k] |= 1ull << bank;
-      auto& w = lane_used[l];
-      if ((size_t)(k >> 6) >= w.size()) w.resize((k >> 6) + 1, 0ull);
-      w[k >> 6] |= 1ull << (k & 63);
       slot_of[l][r] = k;
-      nslots = std::max(nslots, k + 1);
+      lane_at[(size_t)l * D + k] = r;
+      bank_at[(size_t)b * D + k] = (l << 16) | r;
+    }
+  return D;
+}
+
+// Per-tile LDS layout of the table parts (32-bank model, whole-frame waves, <= 2 parts).
+// A 16-byte direct-to-LDS load writes, for lane l, entry group l % 8 (4 entries) of LDS row
+// 8c + l / 8 (32 entries) of chunk c, from any 16-byte group of the same 128-byte table row
+// the lane addresses.  The waves of a tile load the chunks c == wid (mod NW) of every part
+// (c < kChunks - 1; the last chunk, 4-byte loads, keeps the identity), so one permutation
+// of the 8 groups per (part, wave, row in chunk) costs each lane one fixed source offset
+// per part: sigma[p][w][rr][k] is the source group of LDS group k, inv its inverse.  A bank
+// is an LDS entry index mod 32: the permutation moves quads of entries between the 8 bank
+// quads.  Chosen greedily per tile to flatten the busiest bank of each half-wave -- a bound
+// of the slot schedule besides the lane's own entry count -- it lowers the tile's slots.
+using Placed = std::vector<std::vector<std::vector<int>>>;  // [part][lane][slot], -1 free
+
+struct TileLayout {
+  int NW = 0;
+  std::vector<uint8_t> sigma, inv;  // [p][w][rr][k]
+  // LDS entry of table entry i (within part p)
+  int map(int p, int i) const {
+    const int r = i >> 5;
+    if (r >= (kChunks - 1) * 8) return i;
+    const int c = r >> 3, rr = r & 7, w = c % NW, g = (i & 31) >> 2;
+    const int k = inv[(((size_t)p * NW + w) * 8 + rr) * 8 + g];
+    return (r << 5) + (k << 2) + (i & 3);
+  }
+};
+
+// Greedy fit of a tile's layout: per part, the (wave, row-in-chunk) classes in descending
+// order of entry count; per class its 8 groups in descending order, each to the free
+// position that minimises (busiest of the 4 banks it lands on over the half-waves, sum of
+// squared per-half-wave maxima, sum of squared loads).  ws: the tile's waves' schedules,
+// entries as identity LDS entries.
+void tile_layout(int NP, int NW, const std::vector<const Placed*>& ws, TileLayout& T) {
+  T.NW = NW;
+  T.sigma.assign((size_t)NP * NW * 64, 0);
+  T.inv.assign((size_t)NP * NW * 64, 0);
+  const int NH = 2 * (int)ws.size();         // half-waves
+  const int nrl = (kChunks - 1) * 8;         // rows with a layout
+  std::vector<int> A((size_t)NW * 8 * NH * 32), L((size_t)NH * 32), rmax(NH);
+  for (int p = 0; p < NP; ++p) {
+    std::fill(A.begin(), A.end(), 0);
+    std::fill(L.begin(), L.end(), 0);
+    for (int wi = 0; wi < (int)ws.size(); ++wi)
+      for (int lane = 0; lane < 64; ++lane) {
+        const int hw = 2 * wi + lane / 32;
+        for (int v : (*ws[wi])[p][lane]) {
+          if (v < 0) continue;
+          const int r = v >> 5;
+          if (r >= nrl) {
+            ++L[(size_t)hw * 32 + (v & 31)];
+          } else {
+            const int cls = ((r >> 3) % NW) * 8 + (r & 7);
+            ++A[((size_t)cls * NH + hw) * 32 + (v & 31)];
+          }
+        }
+      }
+    for (int hw = 0; hw < NH; ++hw)
+      rmax[hw] = *std::max_element(L.begin() + (size_t)hw * 32, L.begin() + (size_t)hw * 32 + 32);
+    std::vector<int64_t> tot(NW * 8, 0);
+    for (int cls = 0; cls < NW * 8; ++cls)
+      for (size_t q = 0; q < (size_t)NH * 32; ++q) tot[cls] += A[(size_t)cls * NH * 32 + q];
+    std::vector<int> cord(NW * 8);
+    std::iota(cord.begin(), cord.end(), 0);
+    std::stable_sort(cord.begin(), cord.end(), [&](int a, int b) { return tot[a] > tot[b]; });
+    for (int cls : cord) {
+      const int* a = A.data() + (size_t)cls * NH * 32;
+      uint8_t* sg = T.sigma.data() + ((size_t)p * NW * 8 + cls) * 8;
+      uint8_t* iv = T.inv.data() + ((size_t)p * NW * 8 + cls) * 8;
+      int64_t gt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int hw = 0; hw < NH; ++hw)
+        for (int col = 0; col < 32; ++col) gt[col >> 2] += a[hw * 32 + col];
+      int gord[8];
+      std::iota(gord, gord + 8, 0);
+      std::stable_sort(gord, gord + 8, [&](int x, int y) { return gt[x] > gt[y]; });
+      bool used[8] = {false, false, false, false, false, false, false, false};
+      for (int gi : gord) {
+        int best = -1;
+        int64_t b1 = 0, b2 = 0, b3 = 0;
+        for (int pos = 0; pos < 8; ++pos) {
+          if (used[pos]) continue;
+          int64_t m1 = 0, m2 = 0, m3 = 0;
+          for (int hw = 0; hw < NH; ++hw) {
+            int mx = rmax[hw];
+            for (int j = 0; j < 4; ++j) {
+              const int l0 = L[(size_t)hw * 32 + pos * 4 + j], g = a[hw * 32 + gi * 4 + j];
+              const int v = l0 + g;
+              m1 = std::max<int64_t>(m1, v);
+              mx = std::max(mx, v);
+              m3 += (int64_t)2 * l0 * g + (int64_t)g * g;
+            }
+            m2 += (int64_t)mx * mx;
+          }
+          if (best < 0 || m1 < b1 || (m1 == b1 && (m2 < b2 || (m2 == b2 && m3 < b3)))) {
+            best = pos;
+            b1 = m1;
+            b2 = m2;
+            b3 = m3;
+          }
+        }
+        used[best] = true;
+        sg[best] = (uint8_t)gi;
+        iv[gi] = (uint8_t)best;
+        for (int hw = 0; hw < NH; ++hw)
+          for (int j = 0; j < 4; ++j) {
+            int& l0 = L[(size_t)hw * 32 + best * 4 + j];
+            l0 += a[hw * 32 + gi * 4 + j];
+            rmax[hw] = std::max(rmax[hw], l0);
+          }
+      }
     }
   }
-  return nslots;
 }
 
 using GatherFn = void (*)(const float*, int, int, int, int, int, int, const uint32_t*,
-                          const double*, const double*, int, Top3*, int);
+                          const double*, const double*, int, Top3*, int, const uint32_t*);
 
 // Gather instantiations: S slot registers per table part and lane, NP parts, NW waves per
 // workgroup so that the S*NP slots + ~30 working VGPRs fit NW/4 waves per SIMD (512
@@ -1061,6 +1224,7 @@ extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
       if (value < 0 || value > (1 << 24)) return fail(kErrSize, "regroup iterations out of range");
       e->regroup_iters = (int)value;
       return kOk;
+    case 7: e->layout = value != 0; return kOk;
     default: return fail(kErrSize, "rsp_emc_set_option: unknown key");
   }
 }
@@ -1130,11 +1294,26 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
       nentries[d] = ne;
     }
   }
-  // sort frames by entry count (descending) into waves of 64
+  // sort frames into waves of 64 by their largest per-part entry count, then by entry
+  // count (both descending): the slots of a wave's part are at least the largest count
+  // of any of its lanes on that part, and frames alike in both parts share waves
+  std::vector<int> maxpart(N, 0);
+  {
+    std::vector<int> pc(e->NP);
+    for (int64_t d = 0; d < N; ++d) {
+      std::fill(pc.begin(), pc.end(), 0);
+      for (int64_t q = ex_off[d]; q < ex_off[d + 1]; ++q) {
+        const int c = (q < ex_off[d] + ex_n1[d]) ? 1 : ex_cnt[q];
+        pc[e->compact_of[ex_pix[q]] / kPartMax] += c;
+      }
+      maxpart[d] = *std::max_element(pc.begin(), pc.end());
+    }
+  }
   std::vector<int32_t> order(N);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return nentries[a] > nentries[b]; });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return maxpart[a] != maxpart[b] ? maxpart[a] > maxpart[b] : nentries[a] > nentries[b];
+  });
   const int NB = e->NB;
   // lanes per conflict group: 32 banks -> half waves, 64 -> full waves, 0 -> no scheduling
   const int G = NB == 0 ? 64 : NB;
@@ -1146,7 +1325,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
     int X;
     int64_t f0, nf;  // frames order[f0 .. f0+nf)
     int need;        // slots per part (max over parts, multiple of 4)
-    std::vector<std::vector<std::vector<int>>> placed;  // [part][lane][slot]
+    Placed placed;   // [part][lane][slot]: LDS entry within the part, -1 free
   };
   std::vector<Wave> waves;
   auto build_wave = [&](int64_t f0, int X, Wave& wv) {
@@ -1237,6 +1416,123 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
     if (c < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs too many slots");
     wcfg[i] = c;
   }
+  // Table layouts (TileLayout): runs of whole-frame waves get a layout fitted to their
+  // frames and are rescheduled on it; a run whose slot need then fits a cheaper
+  // instantiation with its workgroup size keeps it and becomes one workgroup tile of that
+  // class (tiles first in every class, so that they stay aligned to the workgroups); the
+  // other waves keep the identity.
+  std::vector<TileLayout> layouts;
+  std::vector<int> wlay(nw0, -1), wwid(nw0, 0);  // per wave: layout id, wave within tile
+  {
+    const bool use_layout = e->layout && NB == 32 && e->NP <= 2;
+    auto resched = [&](const Wave& wv, const TileLayout& T, Wave& out) {
+      out.X = wv.X;
+      out.f0 = wv.f0;
+      out.nf = wv.nf;
+      out.need = 0;
+      out.placed.assign(e->NP, std::vector<std::vector<int>>(64));
+      for (int p = 0; p < e->NP; ++p) {
+        int ns_max = 0;
+        for (int g0 = 0; g0 < 64; g0 += G) {
+          std::vector<std::vector<int>> ent(G);
+          for (int l = 0; l < G; ++l)
+            for (int v : wv.placed[p][g0 + l])
+              if (v >= 0) ent[l].push_back(T.map(p, v));
+          std::vector<std::vector<int>> slot_of;
+          const int ns = schedule_group(ent, NB, slot_of);
+          ns_max = std::max(ns_max, ns);
+          for (int l = 0; l < G; ++l) {
+            auto& pl = out.placed[p][g0 + l];
+            pl.assign(ns, -1);
+            for (size_t r = 0; r < ent[l].size(); ++r) pl[slot_of[l][r]] = ent[l][r];
+          }
+        }
+        out.need = std::max(out.need, (int)ceil_div(ns_max, 4) * 4);
+      }
+    };
+    struct Unit {
+      int X, cfg;
+      bool tile;
+      std::vector<int64_t> w;
+    };
+    std::vector<Unit> units;
+    std::vector<char> in_tile(nw0, 0);  // by position in worder
+    if (use_layout) {
+      // Candidate tiles: whole-frame waves by their lane floor (the longest lane on any
+      // part, rounded up to 4: no layout takes a wave below it), then by need, in runs of
+      // the workgroup size of the floor's instantiation.  A run keeps its layout when the
+      // tile's need then fits an instantiation of that workgroup size whose slots, over the
+      // tile, are fewer than its waves' own instantiations'.
+      std::vector<int> lfl(nw0, 0);
+      std::vector<int64_t> cand;
+      for (int64_t i = 0; i < nw0; ++i) {
+        const Wave& wv = waves[worder[i]];
+        if (wv.X != 1) continue;
+        int m = 0;
+        for (const auto& pp : wv.placed)
+          for (const auto& pl : pp) {
+            int n = 0;
+            for (int v : pl) n += v >= 0;
+            m = std::max(m, n);
+          }
+        lfl[i] = (int)ceil_div(m, 4) * 4;
+        cand.push_back(i);
+      }
+      std::stable_sort(cand.begin(), cand.end(), [&](int64_t x, int64_t y) {
+        return lfl[x] != lfl[y] ? lfl[x] > lfl[y] : waves[worder[x]].need > waves[worder[y]].need;
+      });
+      for (size_t t0 = 0; t0 < cand.size();) {
+        const int c0 = choose_cfg(e->NP, 1, std::max(lfl[cand[t0]], 8));
+        const int NWt = kGatherCfgs[c0].NW;
+        if (t0 + NWt > cand.size()) break;
+        std::vector<const Placed*> ps;
+        int64_t old_cost = 0;
+        for (int k = 0; k < NWt; ++k) {
+          ps.push_back(&waves[worder[cand[t0 + k]]].placed);
+          old_cost += kGatherCfgs[wcfg[cand[t0 + k]]].S;
+        }
+        TileLayout T;
+        tile_layout(e->NP, NWt, ps, T);
+        std::vector<Wave> nws(NWt);
+        int need = 0;
+        for (int k = 0; k < NWt; ++k) {
+          resched(waves[worder[cand[t0 + k]]], T, nws[k]);
+          need = std::max(need, nws[k].need);
+        }
+        const int c2 = choose_cfg(e->NP, 1, std::max(need, 8));
+        if (c2 >= 0 && kGatherCfgs[c2].NW == NWt &&
+            (int64_t)NWt * kGatherCfgs[c2].S < old_cost) {
+          Unit u{1, c2, true, {}};
+          for (int k = 0; k < NWt; ++k) {
+            const int64_t wi = worder[cand[t0 + k]];
+            waves[wi] = std::move(nws[k]);
+            wlay[wi] = (int)layouts.size();
+            wwid[wi] = k;
+            in_tile[cand[t0 + k]] = 1;
+            u.w.push_back(wi);
+          }
+          layouts.push_back(std::move(T));
+          units.push_back(std::move(u));
+        }
+        t0 += NWt;
+      }
+    }
+    for (int64_t i = 0; i < nw0; ++i)
+      if (!in_tile[i]) units.push_back(Unit{waves[worder[i]].X, wcfg[i], false, {worder[i]}});
+    // lane split descending, then capacity descending (the order above), tiles first
+    std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) {
+      if (a.X != b.X) return a.X > b.X;
+      if (a.cfg != b.cfg) return a.cfg > b.cfg;
+      return a.tile && !b.tile;
+    });
+    int64_t k = 0;
+    for (const Unit& u : units)
+      for (int64_t wi : u.w) {
+        worder[k] = wi;
+        wcfg[k] = u.cfg;
+        ++k;
+      }
+  }
   e->classes.clear();
   int64_t gw_total = 0, slot_total = 0;
   for (int64_t i = 0; i < nw0;) {
@@ -1259,6 +1555,8 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   const int64_t Npad = n_waves * 64;
   std::vector<uint32_t> slots((size_t)slot_total);
   std::vector<int32_t> perm(Npad, -1);
+  std::vector<uint32_t> lay((size_t)2 * Npad);
+  for (int64_t j = 0; j < Npad; ++j) lay[j] = lay[Npad + j] = (uint32_t)(j % 64) * 16u;
   for (const auto& c : e->classes) {
     const GatherCfg& g = kGatherCfgs[c.cfg];
     const int P = g.S * g.NP;
@@ -1270,8 +1568,21 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
       for (int l = 0; l < 64; ++l)
         if (real && l % wv->X == 0 && l / wv->X < wv->nf)
           perm[gw * 64 + l] = order[wv->f0 + l / wv->X];
+      if (real && wlay[worder[c.w_src + lw]] >= 0) {
+        const TileLayout& T = layouts[wlay[worder[c.w_src + lw]]];
+        const int wid = wwid[worder[c.w_src + lw]];
+        for (int p = 0; p < g.NP; ++p)
+          for (int l = 0; l < 64; ++l)
+            lay[(size_t)p * Npad + gw * 64 + l] =
+                (uint32_t)(l >> 3) * 128u +
+                16u * T.sigma[(((size_t)p * T.NW + wid) * 8 + (l >> 3)) * 8 + (l & 7)];
+      }
       for (int p = 0; p < g.NP; ++p) {
         const int part_len = std::min(kPartMax, e->n_scored - p * kPartMax);
+        // zero pads read LDS entries >= part_len (zero in every table): for 32 banks the
+        // first row with no table entry, which every layout leaves zero; for 64 banks the
+        // tail [part_len, kPStride), where a bank with no zero entry there takes another
+        const int zrow = (int)ceil_div(part_len, 32) * 32;
         for (int g0 = 0; g0 < 64; g0 += G) {
           for (int k = 0; k < g.S; ++k) {
             // banks used by real entries in this slot, then conflict-free zero pads
@@ -1293,8 +1604,14 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
                   int bank = 0;
                   while (bank < NB && ((used >> bank) & 1ull)) ++bank;
                   if (bank == NB) bank = 0;
+                  if (NB == 32) {
+                    entry = zrow + bank;
+                  } else {
+                    entry = part_len + ((bank - part_len) % NB + NB) % NB;
+                    if (entry >= kPStride) entry = part_len + (entry - part_len) % (kPStride - part_len);
+                    bank = entry % NB;
+                  }
                   used |= 1ull << bank;
-                  entry = part_len + ((bank - part_len) % NB + NB) % NB;
                 }
               }
               slots[(size_t)(c.slot_off + ((int64_t)lw * P + p * g.S + k) * 64 + g0 + l)] =
@@ -1308,6 +1625,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   int rc = 0;
   std::vector<double> hb(hbg);
   if ((rc = dev_upload(&e->d_bg, hb)) || (rc = dev_upload(&e->d_slots, slots)) ||
+      (rc = dev_upload(&e->d_lay, lay)) ||
       (rc = dev_upload(&e->d_perm, perm)) ||
       (rc = dev_upload(&e->d_ex_off, ex_off)) || (rc = dev_upload(&e->d_ex_n1, ex_n1)) ||
       (rc = dev_upload(&e->d_ex_pix, ex_pix)) || (rc = dev_upload(&e->d_ex_cnt, ex_cnt)) ||
@@ -1402,7 +1720,7 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
         hipLaunchKernelGGL(gc.fn, dim3((unsigned)(ns * c.n_tiles)), dim3(64 * gc.NW), lds_bytes,
                            st, e->d_tables, e->TS, (int)hb, nb, ns, c.n_tiles, c.gw0,
                            e->d_slots + c.slot_off, e->d_scale_sorted, e->d_init, e->R,
-                           e->d_partials, (int)e->Npad);
+                           e->d_partials, (int)e->Npad, (const uint32_t*)e->d_lay);
       }
       RSP_LAUNCH_CHECK("k_gather");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));

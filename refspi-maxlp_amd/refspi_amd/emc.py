@@ -67,7 +67,7 @@ class EMC:
     def __init__(self, config_file=None, num_streams=4, *, dataset=None, size=None,
                  num_rot=None, shiftx=None, shifty=None, sphere_dia=None, need_scaling=None,
                  output_folder=None, model=None, seed=None, device=0, save=True,
-                 bank_count=None, regroup_iters=None):
+                 bank_count=None, regroup_iters=None, table_layout=None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('EMC needs a GPU (HIP device)')
@@ -77,6 +77,7 @@ class EMC:
         self.save = save
         self.bank_count = bank_count
         self.regroup_iters = regroup_iters
+        self.table_layout = table_layout
         config = configparser.ConfigParser()
         start_dir = '.'
         if config_file is not None:
@@ -168,6 +169,9 @@ class EMC:
             _lib.check(lib.rsp_emc_set_option(h, 3, int(self.bank_count)), 'rsp_emc_set_option')
         if self.regroup_iters is not None:   # swaps of the frame regrouping per window
             _lib.check(lib.rsp_emc_set_option(h, 6, int(self.regroup_iters)),
+                       'rsp_emc_set_option')
+        if self.table_layout is not None:   # per-tile LDS layout of the table parts
+            _lib.check(lib.rsp_emc_set_option(h, 7, int(bool(self.table_layout))),
                        'rsp_emc_set_option')
         d = self.dset
         bg = np.ascontiguousarray(d.bg, dtype='f8')
