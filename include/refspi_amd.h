@@ -88,7 +88,11 @@ int rsp_emc_set_frames(rsp_emc* plan, int64_t num_data, const int32_t* ones,
  *   5 = per-kernel timing with HIP events (1 on; read with rsp_emc_timing)
  *   6 = swaps of the local search that regroups frames into lane groups with flat LDS
  *       bank loads, per window of 8 groups (default 4000; 0 = off; set before frames).
- *       Only the slot layout changes, never the result. */
+ *       Only the slot layout changes, never the result.
+ *   7 = per-workgroup LDS layout of the table parts, fitted to the workgroup's frames
+ *       (1 = on, the default; 0 = identity; 32-bank model and <= 2 parts; set before
+ *       frames).  Only the slot layout and the table loads' addresses change, never the
+ *       result. */
 int rsp_emc_set_option(rsp_emc* plan, int key, int64_t value);
 
 /* Run the E-step for `model_c` (device, npix complex) with per-frame `scales` (device,
