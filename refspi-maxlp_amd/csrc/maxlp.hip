@@ -212,6 +212,15 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
   const int li = (int)(uint32_t)order[blockIdx.x];
   const int t = pixels[li];
   const int64_t e0 = seg_beg[t], e1 = seg_end[t];
+  // issue priority by work: the heaviest pixels (dispatched first) would otherwise share
+  // their SIMDs evenly with the light ones that fill the CU around them and finish last;
+  // with priority the arbiter issues their waves first and the light waves fill the gaps
+  {
+    const int64_t n = e1 - e0;
+    if (n >= 16384) __builtin_amdgcn_s_setprio(3);
+    else if (n >= 4096) __builtin_amdgcn_s_setprio(2);
+    else if (n >= 1024) __builtin_amdgcn_s_setprio(1);
+  }
   const double m1r = mom[4 * t + 0], m1i = mom[4 * t + 1], m2 = mom[4 * t + 2];
   // K-bar = mean_d K_d (exact integer sum)
   {
