@@ -155,6 +155,8 @@ __device__ __forceinline__ double logview_exact(const double* __restrict__ Is,
 // error bound, covers the difference of the two (by four orders of magnitude).
 // --------------------------------------------------------------------------------
 constexpr double kTabLogErr = 1e-9;
+// coefficients 1/(2k+1) of the atanh series, highest first
+__constant__ double kAtanhC[8] = {1. / 15., 1. / 13., 1. / 11., 1. / 9., 1. / 7., 1. / 5., 1. / 3., 1.};
 
 __device__ __forceinline__ double log_table(double v) {
   int e;
@@ -171,14 +173,10 @@ __device__ __forceinline__ double log_table(double v) {
   rc = fma(rc, fma(-dn, rc, 1.), rc);
   const double s = (m - 1.) * rc;
   const double z = s * s;
-  double p = 1. / 15.;
-  p = fma(p, z, 1. / 13.);
-  p = fma(p, z, 1. / 11.);
-  p = fma(p, z, 1. / 9.);
-  p = fma(p, z, 1. / 7.);
-  p = fma(p, z, 1. / 5.);
-  p = fma(p, z, 1. / 3.);
-  p = fma(p, z, 1.);
+  // Horner steps as fma(p, z, c) with the c in scalar registers: one instruction each
+  double p = kAtanhC[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) p = fma(p, z, kAtanhC[k]);
   const double ln2_hi = 0x1.62e42feep-1, ln2_lo = 0x1.a39ef35793c76p-33;
   return fma((double)e, ln2_hi, fma((double)e, ln2_lo, 2. * s * p));
 }
@@ -214,21 +212,23 @@ __global__ __launch_bounds__(256) void k_tables(
   const int part = i / kPartMax;
   const int64_t col = (int64_t)part * PSTRIDE + (i - part * kPartMax);
   const int h0 = hb + blockIdx.y * HC;
-  const int h1 = min(hb + nb, h0 + HC);
-  float emax = 0.f, amax = 0.f;
+  const int h1 = min(hb + nb, h0 + HC);  // > h0: the grid covers [hb, hb + nb) in HC steps
+  double emax_d = 0.;
+  float amax = 0.f;
   for (int h = h0; h < h1; ++h) {
     const int s = h / R, r = h - s * R;
     const double lv = logview_table(I + (int64_t)s * npix, bg, size, t, rcos[r], rsin[r]);
     const float f = (float)lv;
     tables[(int64_t)(h - hb) * TS + col] = f;
-    // the error against the exact path's log-view, rounded up to float: the FP32 rounding
-    // plus the two logs' difference
-    const double d = fabs((double)f - lv) + kTabLogErr;
-    float df = (float)d;
-    if ((double)df < d) df = nextafterf(df, INFINITY);
-    emax = fmaxf(emax, df);
+    emax_d = fmax(emax_d, fabs((double)f - lv));
     amax = fmaxf(amax, fabsf(f));
   }
+  // the error against the exact path's log-view, rounded up to float: the FP32 rounding
+  // plus the two logs' difference.  x -> roundup_f32(x + kTabLogErr) is monotone, so the
+  // largest rounded bound is the rounded bound of the largest rounding error.
+  const double d = emax_d + kTabLogErr;
+  float emax = (float)d;
+  if ((double)emax < d) emax = nextafterf(emax, INFINITY);
   // max of non-negative floats == max of their bit patterns: order independent
   atomicMax(errmax + i, __float_as_uint(emax));
   atomicMax(absmax + i, __float_as_uint(amax));
