@@ -181,17 +181,16 @@ __device__ __forceinline__ double log_table(double v) {
   return fma((double)e, ln2_hi, fma((double)e, ln2_lo, 2. * s * p));
 }
 
-// logview_exact with log_table: the same sample v (bit for bit), its log within
-// kTabLogErr of the exact path's
-__device__ __forceinline__ double logview_table(const double* __restrict__ Is,
-                                                const double* __restrict__ bg, int size,
-                                                int t, double ac, double as) {
-  const int x = t / size, y = t - x * size;
+// logview_exact with log_table, for detector pixel (x, y) with background bgt: the same
+// sample v (bit for bit), its log within kTabLogErr of the exact path's
+__device__ __forceinline__ double logview_table(const double* __restrict__ Is, double bgt,
+                                                int size, int x, int y, double ac,
+                                                double as) {
   double tx, ty, v;
   rot_source(x, y, size / 2, ac, as, tx, ty);
   if (!bilinear(Is, size, tx, ty, v)) return -1000.;
   v = v * 1.;
-  v = v + bg[t];
+  v = v + bgt;
   return v < 1.e-20 ? -1000. : log_table(v);
 }
 
@@ -213,15 +212,30 @@ __global__ __launch_bounds__(256) void k_tables(
   const int64_t col = (int64_t)part * PSTRIDE + (i - part * kPartMax);
   const int h0 = hb + blockIdx.y * HC;
   const int h1 = min(hb + nb, h0 + HC);  // > h0: the grid covers [hb, hb + nb) in HC steps
+  const int x = t / size, y = t - x * size;
+  const double bgt = bg[t];
+  // (state, rotation) of h stepped along h; the next rotation's cos/sin are loaded one
+  // hypothesis ahead (r < R always, so the load past the last one stays in bounds)
+  int s = h0 / R, r = h0 - s * R;
+  double ac = rcos[r], as = rsin[r];
   double emax_d = 0.;
   float amax = 0.f;
   for (int h = h0; h < h1; ++h) {
-    const int s = h / R, r = h - s * R;
-    const double lv = logview_table(I + (int64_t)s * npix, bg, size, t, rcos[r], rsin[r]);
+    int r2 = r + 1, s2 = s;
+    if (r2 == R) {
+      r2 = 0;
+      ++s2;
+    }
+    const double ac2 = rcos[r2], as2 = rsin[r2];
+    const double lv = logview_table(I + (int64_t)s * npix, bgt, size, x, y, ac, as);
     const float f = (float)lv;
     tables[(int64_t)(h - hb) * TS + col] = f;
     emax_d = fmax(emax_d, fabs((double)f - lv));
     amax = fmaxf(amax, fabsf(f));
+    r = r2;
+    s = s2;
+    ac = ac2;
+    as = as2;
   }
   // the error against the exact path's log-view, rounded up to float: the FP32 rounding
   // plus the two logs' difference.  x -> roundup_f32(x + kTabLogErr) is monotone, so the
