@@ -194,6 +194,42 @@ __device__ __forceinline__ double logview_table(const double* __restrict__ Is, d
   return v < 1.e-20 ? -1000. : log_table(v);
 }
 
+// One hypothesis of the table loop, split so that its image reads can be issued a
+// hypothesis ahead of their use: tab_fetch computes the rotated source position and loads
+// the bilinear cell (from the image's first cell when the position is out of bounds, so
+// that every read stays in bounds and no branch is taken); tab_value forms the sample of
+// bilinear() with the same operations in the same order, adds the background and takes
+// log_table.
+struct TabSample {
+  double p00, p01, p10, p11, fx, fy;
+  bool in;
+};
+
+__device__ __forceinline__ void tab_fetch(const double* __restrict__ Is, int size, int x, int y,
+                                          double ac, double as, TabSample& q) {
+  double tx, ty;
+  rot_source(x, y, size / 2, ac, as, tx, ty);
+  const double fxl = floor(tx), fyl = floor(ty);
+  const int ix = (int)fxl, iy = (int)fyl;
+  q.in = !(ix < 0 || ix > size - 2 || iy < 0 || iy > size - 2);
+  q.fx = tx - ix;
+  q.fy = ty - iy;
+  const double* p = Is + (q.in ? (int64_t)ix * size + iy : 0);
+  q.p00 = p[0];
+  q.p01 = p[1];
+  q.p10 = p[size];
+  q.p11 = p[size + 1];
+}
+
+__device__ __forceinline__ double tab_value(const TabSample& q, double bgt) {
+  if (!q.in) return -1000.;
+  const double cx = 1. - q.fx, cy = 1. - q.fy;
+  double v = q.p00 * cx * cy + q.p10 * q.fx * cy + q.p01 * cx * q.fy + q.p11 * q.fx * q.fy;
+  v = v * 1.;
+  v = v + bgt;
+  return v < 1.e-20 ? -1000. : log_table(v);
+}
+
 // --------------------------------------------------------------------------------
 // 3a: FP32 tables for hypotheses [hb, hb+nb)
 // tables[(h-hb)*TS + part*PSTRIDE + (i - part*kPartMax)] = fl32(logview)
@@ -214,28 +250,43 @@ __global__ __launch_bounds__(256) void k_tables(
   const int h1 = min(hb + nb, h0 + HC);  // > h0: the grid covers [hb, hb + nb) in HC steps
   const int x = t / size, y = t - x * size;
   const double bgt = bg[t];
-  // (state, rotation) of h stepped along h; the next rotation's cos/sin are loaded one
-  // hypothesis ahead (r < R always, so the load past the last one stays in bounds)
-  int s = h0 / R, r = h0 - s * R;
-  double ac = rcos[r], as = rsin[r];
+  // Software pipeline along h: the cell of hypothesis h+1 is loaded before the log of h,
+  // the cos/sin of h+2 before the cell of h+1.  (s1, r1): the (state, rotation) of h+1;
+  // r < R always, so the cos/sin loads past the batch stay in bounds; no cell is loaded
+  // past the batch's last hypothesis (its state may not exist).
+  // Two samples in turn (q0, q1: no copies between iterations, so the wait for one
+  // hypothesis's cell leaves the next one's reads in flight).
+  int s1 = h0 / R, r1 = h0 - s1 * R;
+  TabSample q0, q1;
+  tab_fetch(I + (int64_t)s1 * npix, size, x, y, rcos[r1], rsin[r1], q0);
+  if (++r1 == R) {
+    r1 = 0;
+    ++s1;
+  }
+  double ac1 = rcos[r1], as1 = rsin[r1];
   double emax_d = 0.;
   float amax = 0.f;
-  for (int h = h0; h < h1; ++h) {
-    int r2 = r + 1, s2 = s;
+  auto step = [&](int h, const TabSample& cur, TabSample& nxt) {
+    int r2 = r1 + 1, s2 = s1;
     if (r2 == R) {
       r2 = 0;
       ++s2;
     }
     const double ac2 = rcos[r2], as2 = rsin[r2];
-    const double lv = logview_table(I + (int64_t)s * npix, bgt, size, x, y, ac, as);
+    if (h + 1 < h1) tab_fetch(I + (int64_t)s1 * npix, size, x, y, ac1, as1, nxt);
+    const double lv = tab_value(cur, bgt);
     const float f = (float)lv;
     tables[(int64_t)(h - hb) * TS + col] = f;
     emax_d = fmax(emax_d, fabs((double)f - lv));
     amax = fmaxf(amax, fabsf(f));
-    r = r2;
-    s = s2;
-    ac = ac2;
-    as = as2;
+    s1 = s2;
+    r1 = r2;
+    ac1 = ac2;
+    as1 = as2;
+  };
+  for (int h = h0; h < h1; h += 2) {
+    step(h, q0, q1);
+    if (h + 1 < h1) step(h + 1, q1, q0);
   }
   // the error against the exact path's log-view, rounded up to float: the FP32 rounding
   // plus the two logs' difference.  x -> roundup_f32(x + kTabLogErr) is monotone, so the
