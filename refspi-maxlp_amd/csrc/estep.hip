@@ -1590,13 +1590,40 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
       if (a.cfg != b.cfg) return a.cfg > b.cfg;
       return a.tile && !b.tile;
     });
-    int64_t k = 0;
-    for (const Unit& u : units)
-      for (int64_t wi : u.w) {
-        worder[k] = wi;
-        wcfg[k] = u.cfg;
-        ++k;
+    // Runs of one instantiation (the classes): tiles, then single waves.  The last
+    // workgroup of a class would be completed with empty waves, each reading its slots as
+    // zero pads; it takes the neediest single waves of the following classes with the
+    // same lane split instead (their capacity is smaller, so they fit).
+    struct Run {
+      int X, cfg;
+      std::vector<int64_t> tiles, singles;
+    };
+    std::vector<Run> runs;
+    for (const Unit& u : units) {
+      if (runs.empty() || runs.back().cfg != u.cfg) runs.push_back(Run{u.X, u.cfg, {}, {}});
+      auto& dst = u.tile ? runs.back().tiles : runs.back().singles;
+      dst.insert(dst.end(), u.w.begin(), u.w.end());
+    }
+    for (size_t a = 0; a < runs.size(); ++a) {
+      const int NWa = kGatherCfgs[runs[a].cfg].NW;
+      const int64_t n = (int64_t)(runs[a].tiles.size() + runs[a].singles.size());
+      int64_t short_by = (NWa - n % NWa) % NWa;
+      for (size_t b = a + 1; short_by > 0 && b < runs.size() && runs[b].X == runs[a].X; ++b) {
+        auto& src = runs[b].singles;
+        const int64_t take = std::min<int64_t>(short_by, (int64_t)src.size());
+        runs[a].singles.insert(runs[a].singles.end(), src.begin(), src.begin() + take);
+        src.erase(src.begin(), src.begin() + take);
+        short_by -= take;
       }
+    }
+    int64_t k = 0;
+    for (const Run& r : runs)
+      for (const auto* part : {&r.tiles, &r.singles})
+        for (int64_t wi : *part) {
+          worder[k] = wi;
+          wcfg[k] = r.cfg;
+          ++k;
+        }
   }
   e->classes.clear();
   int64_t gw_total = 0, slot_total = 0;
