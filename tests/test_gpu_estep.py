@@ -172,6 +172,36 @@ def test_estep_table_layout_does_not_change_result():
     assert out[1][2]['lds_read_bytes_per_hyp'] < out[0][2]['lds_read_bytes_per_hyp']
 
 
+def test_estep_bank_models_full_parts():
+    """The 64-bank and unscheduled slot layouts on 185^2 frames, whose first table part is
+    full (12,512 entries): the zero pads of a full part stay inside its LDS buffer, and
+    argmax and FP64 log-likelihood equal the 32-bank plan's bit for bit."""
+    import types
+    import torch
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.emc import EMC
+    cfg = synth.SynthConfig(**{**synth.CONFIG_185.__dict__, 'num_data': 3000, 'seed': 9})
+    d = synth.make_frames_torch(cfg, device='cuda')
+    fr = types.SimpleNamespace(num_pix=d['num_pix'], bg=None, **{
+        k: d[k].cpu().numpy().astype('i4')
+        for k in ('ones', 'multi', 'place_ones', 'place_multi', 'count_multi')})
+    model = synth.model_scales(cfg)[0] * synth.object_ft(cfg)
+    grid = dict(shiftx=(-1., 1., 3), shifty=(-1., 1., 3), sphere_dia=(7., 9., 2))
+    out = {}
+    for banks in (32, 64, 0):
+        emc = EMC(dataset=Dataset(frames=fr), size=cfg.size, num_rot=24, model=model, seed=1,
+                  save=False, bank_count=banks, **grid)
+        emc._calculate_prob(torch.as_tensor(emc.model, device='cuda'))
+        emc._get_rmax()
+        assert emc.estep_stats()['table_parts'] == 2
+        out[banks] = (emc.rmax.copy(), emc.maxll.copy())
+        del emc
+    for banks in (64, 0):
+        assert np.array_equal(out[banks][0], out[32][0]), banks
+        np.testing.assert_array_equal(out[banks][1], out[32][1])
+
+
 def test_estep_batching_and_segments_do_not_change_result(small_frames, small_model):
     import torch
     m = torch.as_tensor(small_model, device='cuda')
