@@ -1749,12 +1749,15 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
     RSP_HIP(hipMemsetAsync(e->d_tables, 0, sizeof(float) * batch * e->TS, st));
     e->batch = batch;
   }
-  // segments per batch and slot class: each class launch gets about 4 workgroups per CU
-  // (a class with few tiles gets more hypothesis segments), so that every launch fills the
-  // 256 CUs and the classes take time in proportion to their frames
+  // segments per batch and slot class: at most 128 hypotheses per workgroup, and at least
+  // about 4 workgroups per CU in every class launch (a class with few tiles gets more
+  // hypothesis segments), so that every launch fills the 256 CUs.  Measured at 185^2 with
+  // 100k frames (6 classes of 4 to 40 tiles): 4 workgroups per CU alone 1.86 s per E-step,
+  // a fixed 64 / 128 / 256 / 512 segments 1.87 / 1.81 / 1.83 / 1.92 s.
   if ((int)e->classes.size() > kMaxClass) return fail(kErrUnsupported, "too many slot classes");
   auto nseg_of = [&](const rsp_emc::Class& c, int64_t nb) {
-    int64_t ns = e->opt_nseg > 0 ? e->opt_nseg : ceil_div(4 * 256, c.n_tiles);
+    int64_t ns = e->opt_nseg > 0 ? e->opt_nseg
+                                 : std::max(ceil_div(4 * 256, c.n_tiles), ceil_div(nb, 128));
     ns = std::max<int64_t>(1, std::min<int64_t>(ns, nb));
     if (ns >= 8 && e->opt_nseg == 0) ns = ns / 8 * 8;
     return (int)ns;
