@@ -256,8 +256,12 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       __syncthreads();
       for (int i = threadIdx.x; i < n; i += kThreads) sh[i] = ent[cb + i];
       __syncthreads();
+      // entry i+1 is read from LDS before the arithmetic of entry i, so that the read's
+      // latency is not waited for on every entry
+      Entry En = sh[0];
       for (int i = 0; i < n; ++i) {
-        const Entry E = sh[i];
+        const Entry E = En;
+        if (i + 1 < n) En = sh[i + 1];
         const int K = (int)E.k;
         if (K == 1) {  // most entries: straight-line code over the candidates
 #pragma unroll
