@@ -1783,7 +1783,8 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
   RSP_HIP(hipMemsetAsync(e->d_counters, 0, sizeof(unsigned) * 4, st));
   const size_t lds_bytes = 0;  // the gather's two part buffers are static LDS
   if (!e->exact_mode) {
-    const int HC = 64;
+    const int HC = 32;  // hypotheses per k_tables thread (same-process A/B: 8, 16, 32, 64,
+                        // 128 -> 180, 168, 155-162, 163, 176 ms per iteration)
     int first = 1;
     const int64_t nbatch = ceil_div(e->H, batch);
     if (e->timing) {
