@@ -1781,7 +1781,7 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
   RSP_HIP(hipMemsetAsync(e->d_errmax, 0, sizeof(unsigned) * e->n_scored, st));
   RSP_HIP(hipMemsetAsync(e->d_absmax, 0, sizeof(unsigned) * e->n_scored, st));
   RSP_HIP(hipMemsetAsync(e->d_counters, 0, sizeof(unsigned) * 4, st));
-  const size_t lds_bytes = 0;  // the gather's two part buffers are static LDS
+  const size_t lds_bytes = 0;  // the gather's three part buffers are static LDS
   if (!e->exact_mode) {
     const int HC = 32;  // hypotheses per k_tables thread (same-process A/B: 8, 16, 32, 64,
                         // 128 -> 180, 168, 155-162, 163, 176 ms per iteration)
