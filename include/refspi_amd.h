@@ -117,6 +117,14 @@ int rsp_emc_timing(const rsp_emc* plan, double* out);
 
 int rsp_emc_destroy(rsp_emc* plan);
 
+/* Host-side check of the photon slot scheduler and the per-workgroup table layouts (no
+ * device calls; no counterpart in the reference: the reference has no slot schedule).
+ * `trials` random lane x bank groups and trials/500 (at least 1) rounds of random tiles,
+ * from `seed`.  out (host int64[5]): [0] groups, [1] slot conflicts, [2] groups whose slot
+ * count exceeds the maximum degree, [3] layouts, [4] layout faults; [1], [2], [4] are 0
+ * for a correct scheduler. */
+int rsp_emc_schedule_check(int64_t trials, uint64_t seed, int64_t* out);
+
 /* ---------------------------------------------------------------------------------
  * MaxLP plan: replaces maxLP.py:304-340 (MaxLPhaser._run_phaser) for one GPU.
  * ------------------------------------------------------------------------------- */

@@ -1881,6 +1881,107 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
   return kOk;
 }
 
+// Host-side check of the photon slot scheduler and the table layouts (no device calls):
+// `trials` random lane x bank groups (32 and 64 banks, uniform and clustered entries)
+// through schedule_group, and random tiles (1-2 parts, 8/12/16 waves) through tile_layout.
+// out[0] groups, out[1] conflicts (a lane or bank twice in one slot, an entry without a
+// valid slot), out[2] groups whose slot count is not the maximum degree, out[3] layouts,
+// out[4] layout faults (an LDS row that is not a permutation of its table row, sigma and
+// inv not inverse, a direct-to-LDS source offset outside its chunk or not landing where
+// map() puts the entry).
+extern "C" int rsp_emc_schedule_check(int64_t trials, uint64_t seed, int64_t* out) {
+  if (!out) return fail(kErrNull, "rsp_emc_schedule_check: null pointer");
+  if (trials < 0) return fail(kErrSize, "rsp_emc_schedule_check: trials < 0");
+  uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+  auto rnd = [&](uint64_t n) {  // splitmix64
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (z ^ (z >> 31)) % n;
+  };
+  for (int i = 0; i < 5; ++i) out[i] = 0;
+  for (int64_t it = 0; it < trials; ++it) {
+    const int NB = it % 3 == 0 ? 64 : 32, G = NB;
+    const int mean = 1 + (int)rnd(40);
+    const bool clustered = it % 5 == 0;
+    std::vector<std::vector<int>> ent(G);
+    for (int l = 0; l < G; ++l) {
+      const int n = (int)rnd(2 * mean);
+      for (int k = 0; k < n; ++k)
+        ent[l].push_back(clustered ? (int)rnd(7) * 32 : (int)rnd(kPartMax));
+    }
+    std::vector<std::vector<int>> so;
+    const int ns = schedule_group(ent, NB, so);
+    int D = 0;
+    std::vector<int> bd(NB, 0);
+    for (int l = 0; l < G; ++l) {
+      D = std::max(D, (int)ent[l].size());
+      for (int v : ent[l]) ++bd[v % NB];
+    }
+    for (int x : bd) D = std::max(D, x);
+    out[2] += ns != D;
+    std::vector<char> lu((size_t)G * (ns + 1), 0), bu((size_t)NB * (ns + 1), 0);
+    for (int l = 0; l < G; ++l)
+      for (size_t r = 0; r < ent[l].size(); ++r) {
+        const int k = (l < (int)so.size() && r < so[l].size()) ? so[l][r] : -1;
+        if (k < 0 || k >= ns) {
+          ++out[1];
+          continue;
+        }
+        out[1] += lu[(size_t)l * (ns + 1) + k]++ > 0;
+        out[1] += bu[(size_t)(ent[l][r] % NB) * (ns + 1) + k]++ > 0;
+      }
+    ++out[0];
+  }
+  const int64_t ntiles = std::max<int64_t>(1, trials / 500);
+  for (int64_t it = 0; it < ntiles; ++it)
+    for (int NP = 1; NP <= 2; ++NP)
+      for (int NW : {16, 12, 8}) {
+        std::vector<Placed> pl(NW);
+        for (auto& P : pl) {
+          P.assign(NP, std::vector<std::vector<int>>(64));
+          for (int p = 0; p < NP; ++p)
+            for (int l = 0; l < 64; ++l)
+              for (int k = 0; k < 30; ++k)
+                P[p][l].push_back(rnd(3) == 0 ? -1 : (int)rnd(kPartMax));
+        }
+        std::vector<const Placed*> ps;
+        for (auto& P : pl) ps.push_back(&P);
+        TileLayout T;
+        tile_layout(NP, NW, ps, T);
+        int64_t bad = 0;
+        for (int p = 0; p < NP; ++p) {
+          for (int r = 0; r < kChunks * 8; ++r) {
+            int seen[32] = {0};
+            for (int i = r * 32; i < r * 32 + 32; ++i) {
+              const int m = T.map(p, i);
+              if (m / 32 != r) ++bad;
+              else ++seen[m % 32];
+            }
+            for (int x : seen) bad += x != 1;
+          }
+          for (int q = 0; q < NW * 64; ++q) {
+            const uint8_t* sg = &T.sigma[((size_t)p * NW * 8 + q / 8) * 8];
+            const uint8_t* iv = &T.inv[((size_t)p * NW * 8 + q / 8) * 8];
+            bad += sg[iv[q % 8]] != q % 8;
+          }
+          for (int wid = 0; wid < NW; ++wid)
+            for (int c = wid; c < kChunks - 1; c += NW)
+              for (int l = 0; l < 64; ++l) {
+                const unsigned off = (unsigned)(l >> 3) * 128u +
+                    16u * T.sigma[(((size_t)p * NW + wid) * 8 + (l >> 3)) * 8 + (l & 7)];
+                if (off + 16 > 1024) ++bad;
+                for (int j = 0; j < 4; ++j)
+                  bad += T.map(p, c * 256 + (int)off / 4 + j) != c * 256 + 4 * l + j;
+              }
+        }
+        ++out[3];
+        out[4] += bad;
+      }
+  return kOk;
+}
+
 extern "C" int rsp_emc_stats(const rsp_emc* e, int64_t* out) {
   if (!e || !out) return fail(kErrNull, "rsp_emc_stats: null pointer");
   for (int i = 0; i < 8; ++i) out[i] = e->stats[i];

@@ -35,6 +35,7 @@ _SIGS = {
     'rsp_emc_estep': (C.c_int, [vp, vp, vp, vp, vp, vp, f64p, vp]),
     'rsp_emc_stats': (C.c_int, [vp, i64p]),
     'rsp_emc_timing': (C.c_int, [vp, f64p]),
+    'rsp_emc_schedule_check': (C.c_int, [C.c_int64, C.c_uint64, i64p]),
     'rsp_emc_destroy': (C.c_int, [vp]),
     'rsp_maxlp_create': (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
     'rsp_maxlp_set_frames': (C.c_int, [vp, C.c_int64, i32p, i32p, i32p, i32p, i32p]),
