@@ -149,14 +149,16 @@ __device__ __forceinline__ double logview_exact(const double* __restrict__ Is,
 // The FP64 natural log of the tables: Cody-Waite argument reduction v = 2^e m with
 // m in [sqrt(1/2), sqrt(2)) and ln 2 split into hi (32 significant bits: e * hi is exact)
 // + lo, and log m = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716, by its series up to
-// s^15 (truncation < 1.3e-14).  Measured against an 80-bit log (with a reciprocal start
-// 2^-23 off): |error| <= 6.9e-14 for v in [1e-20, 1e300] (1.2e-14 for v in [0.5, 2)).  About a quarter of the instructions of
-// the library log, which the exact path keeps; kTabLogErr, added to every table entry's
-// error bound, covers the difference of the two (by four orders of magnitude).
+// s^11 (truncation < 1.8e-11), s through the hardware reciprocal and one Newton step.
+// Measured against an 80-bit log (with the reciprocal start 2^-23 off either way):
+// |error| <= 1.8e-11 for v in [1e-20, 1e300].  About a fifth of the instructions of the
+// library log, which the exact path keeps; kTabLogErr, added to every table entry's
+// error bound, covers the difference of the two (by a factor of 50) and is itself far
+// below the FP32 rounding of the entries (~5e-8 relative).
 // --------------------------------------------------------------------------------
 constexpr double kTabLogErr = 1e-9;
 // coefficients 1/(2k+1) of the atanh series, highest first
-__constant__ double kAtanhC[8] = {1. / 15., 1. / 13., 1. / 11., 1. / 9., 1. / 7., 1. / 5., 1. / 3., 1.};
+__constant__ double kAtanhC[6] = {1. / 11., 1. / 9., 1. / 7., 1. / 5., 1. / 3., 1.};
 
 __device__ __forceinline__ double log_table(double v) {
   int e;
@@ -165,18 +167,17 @@ __device__ __forceinline__ double log_table(double v) {
     m *= 2.;
     --e;
   }
-  // s = (m - 1)/(m + 1) through the hardware reciprocal and two Newton steps (a start
-  // 2^-23 off ends within FP64 rounding)
+  // s = (m - 1)/(m + 1) through the hardware reciprocal and one Newton step (a start
+  // 2^-23 off ends within 2^-45)
   const double dn = m + 1.;
   double rc = __builtin_amdgcn_rcp(dn);
-  rc = fma(rc, fma(-dn, rc, 1.), rc);
   rc = fma(rc, fma(-dn, rc, 1.), rc);
   const double s = (m - 1.) * rc;
   const double z = s * s;
   // Horner steps as fma(p, z, c) with the c in scalar registers: one instruction each
   double p = kAtanhC[0];
 #pragma unroll
-  for (int k = 1; k < 8; ++k) p = fma(p, z, kAtanhC[k]);
+  for (int k = 1; k < 6; ++k) p = fma(p, z, kAtanhC[k]);
   const double ln2_hi = 0x1.62e42feep-1, ln2_lo = 0x1.a39ef35793c76p-33;
   return fma((double)e, ln2_hi, fma((double)e, ln2_lo, 2. * s * p));
 }
