@@ -139,17 +139,19 @@ def test_estep_frame_regrouping_does_not_change_result(small_frames, small_model
         np.testing.assert_array_equal(l, out[0][1])
 
 
-def test_estep_table_layout_does_not_change_result():
+@pytest.mark.parametrize('size,parts', [(185, 2), (128, 1)])
+def test_estep_table_layout_does_not_change_result(size, parts):
     """Option 7 (per-tile LDS layout of the table parts) changes only the slot layout and
     the table loads' addresses: argmax and FP64 log-likelihood are identical with and
-    without it, on 185^2 frames (two table parts) numerous enough for full workgroup tiles,
-    and the layout lowers the gather's slot count."""
+    without it, on frames numerous enough for full workgroup tiles (185^2: two table
+    parts; 128^2: one), and the layout lowers the gather's slot count."""
     import types
     import torch
     from refspi_amd import synth
     from refspi_amd.dset import Dataset
     from refspi_amd.emc import EMC
-    cfg = synth.SynthConfig(**{**synth.CONFIG_185.__dict__, 'num_data': 6000, 'seed': 5})
+    cfg = synth.SynthConfig(**{**synth.CONFIG_185.__dict__, 'num_data': 6000, 'seed': 5,
+                               'size': size})
     d = synth.make_frames_torch(cfg, device='cuda')
     fr = types.SimpleNamespace(num_pix=d['num_pix'], bg=None, **{
         k: d[k].cpu().numpy().astype('i4')
@@ -166,7 +168,7 @@ def test_estep_table_layout_does_not_change_result():
         del emc
     print('\nLDS read bytes per hypothesis: identity %.0f, layout %.0f' %
           (out[0][2]['lds_read_bytes_per_hyp'], out[1][2]['lds_read_bytes_per_hyp']))
-    assert out[1][3]['table_parts'] == 2
+    assert out[1][3]['table_parts'] == parts
     assert np.array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
     assert out[1][2]['lds_read_bytes_per_hyp'] < out[0][2]['lds_read_bytes_per_hyp']
