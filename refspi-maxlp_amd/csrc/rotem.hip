@@ -176,7 +176,7 @@ __device__ __forceinline__ double sphere_guarded(double s) {
 // R_d(j) rows of frames [d0, d0 + gridDim.x): B[(d - d0) * Kp + 2j], normR[d - d0].
 template <int MODE>
 __global__ __launch_bounds__(256) void k_ref_rows(
-    int size, int K, const int32_t* __restrict__ scored, const int32_t* __restrict__ jrad,
+    int size, int K, const uint32_t* __restrict__ pxy, const int32_t* __restrict__ jrad,
     const double* __restrict__ rad, int nrad, double rel, const double* __restrict__ sx,
     const double* __restrict__ sy, const double* __restrict__ dia, int64_t d0, int64_t N,
     void* __restrict__ B, int64_t Bpad, int64_t Kp, double* __restrict__ normR) {
@@ -211,9 +211,8 @@ __global__ __launch_bounds__(256) void k_ref_rows(
       vr[u] = 0.;
       vi[u] = 0.;
       if (j < K) {
-        const int t = scored[j];
-        const int x = t / size, y = t - (t / size) * size;
-        const double2 a = ex[x], b = ey[y];
+        const uint32_t q = pxy[j];
+        const double2 a = ex[q & 0xffffu], b = ey[q >> 16];
         const double pr = a.x * b.x - a.y * b.y, pi = a.x * b.y + a.y * b.x;
         const double sv = S[jrad[j]];
         vr[u] = sv * pr;
@@ -833,6 +832,7 @@ struct rsp_rot {
   std::vector<int32_t> scored, compact_of;
   double *d_rcos = nullptr, *d_rsin = nullptr, *d_rad = nullptr;
   int32_t *d_scored = nullptr, *d_jrad = nullptr, *d_compact_of = nullptr;
+  uint32_t* d_pxy = nullptr;  // scored pixel j as x | y << 16 (no division per pixel)
   // resident frames
   int64_t N = 0, nnz = 0;
   int64_t *d_fo = nullptr, *d_po = nullptr;
@@ -1014,7 +1014,7 @@ int launch_ref_to(rsp_rot* p, const double* sx, const double* sy, const double* 
   const size_t shmem =
       sizeof(double) * (size_t)((p->nrad + 1) & ~1) + 2 * sizeof(double2) * (size_t)p->size;
   hipLaunchKernelGGL(k_ref_rows<MODE>, dim3((unsigned)n), dim3(256), shmem, st, p->size, p->K,
-                     p->d_scored, p->d_jrad, p->d_rad, p->nrad, p->rel, sx, sy, dia, d0, N, B,
+                     p->d_pxy, p->d_jrad, p->d_rad, p->nrad, p->rel, sx, sy, dia, d0, N, B,
                      Bpad, p->Kp, normR);
   RSP_LAUNCH_CHECK("k_ref_rows");
   return kOk;
@@ -1401,13 +1401,17 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
   }
   std::vector<int32_t> jrad(p->K);
   for (int j = 0; j < p->K; ++j) jrad[j] = radix[r2_of[j]];
+  std::vector<uint32_t> pxy(p->K);
+  for (int j = 0; j < p->K; ++j)
+    pxy[j] = (uint32_t)(p->scored[j] / size) | ((uint32_t)(p->scored[j] % size) << 16);
   p->nrad = (int)rad.size();
   p->Kp = xterm_pad_k(2 * (int64_t)p->K);
   std::vector<double> rc(rot_cos, rot_cos + num_rot), rs(rot_sin, rot_sin + num_rot);
   int rc_ = kOk;
   if ((rc_ = dupload(&p->d_rcos, rc)) || (rc_ = dupload(&p->d_rsin, rs)) ||
       (rc_ = dupload(&p->d_rad, rad)) || (rc_ = dupload(&p->d_scored, p->scored)) ||
-      (rc_ = dupload(&p->d_jrad, jrad)) || (rc_ = dupload(&p->d_compact_of, p->compact_of))) {
+      (rc_ = dupload(&p->d_jrad, jrad)) || (rc_ = dupload(&p->d_compact_of, p->compact_of)) ||
+      (rc_ = dupload(&p->d_pxy, pxy))) {
     rsp_rot_destroy(p);
     return rc_;
   }
@@ -1722,6 +1726,7 @@ int rsp_rot_destroy(rsp_rot* p) {
   dfree(p->d_rsin);
   dfree(p->d_rad);
   dfree(p->d_scored);
+  dfree(p->d_pxy);
   dfree(p->d_jrad);
   dfree(p->d_compact_of);
   dfree(p->d_fo);
