@@ -649,7 +649,12 @@ __global__ __launch_bounds__(256) void k_adj_frame(
       const double zr = o.x + Rv.x, zi = o.y + Rv.y;
       const double n2 = zr * zr + zi * zi;
       if (sc * n2 >= 1.e-20) {
-        const double f = k / n2;
+        // k / n2 through the hardware reciprocal and two Newton steps (within about an ulp
+        // of the division, in fewer instructions: this loop runs per photon x hypothesis)
+        double rn = __builtin_amdgcn_rcp(n2);
+        rn = fma(rn, fma(-n2, rn, 1.), rn);
+        rn = fma(rn, fma(-n2, rn, 1.), rn);
+        const double f = k * rn;
         ax[t] += f * (zr * Dx.x + zi * Dx.y);
         ay[t] += f * (zr * Dy.x + zi * Dy.y);
         ad[t] += f * (zr * Dd.x + zi * Dd.y);
