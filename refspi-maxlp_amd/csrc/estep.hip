@@ -1754,7 +1754,8 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
   // about 4 workgroups per CU in every class launch (a class with few tiles gets more
   // hypothesis segments), so that every launch fills the 256 CUs.  Measured at 185^2 with
   // 100k frames (6 classes of 4 to 40 tiles): 4 workgroups per CU alone 1.86 s per E-step,
-  // a fixed 64 / 128 / 256 / 512 segments 1.87 / 1.81 / 1.83 / 1.92 s.
+  // a fixed 64 / 96 / 128 / 160 / 192 / 256 / 512 segments 1.87 / 1.87 / 1.81 / 1.81 /
+  // 1.83 / 1.83 / 1.92 s, this rule 1.80 s.
   if ((int)e->classes.size() > kMaxClass) return fail(kErrUnsupported, "too many slot classes");
   auto nseg_of = [&](const rsp_emc::Class& c, int64_t nb) {
     int64_t ns = e->opt_nseg > 0 ? e->opt_nseg
