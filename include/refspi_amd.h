@@ -9,8 +9,9 @@
  *    marked "host".  Sizes are int64_t, scalars double.
  *  - Calls that take an rsp_stream_t (a hipStream_t; NULL = default stream) are
  *    asynchronous on that stream unless stated otherwise.
- *  - Plans (rsp_emc, rsp_maxlp) own their scratch device memory; one plan per host
- *    thread at a time (thread-compatible, not thread-safe).
+ *  - Plans (rsp_emc, rsp_maxlp, rsp_rot) own their scratch device memory and, for
+ *    rsp_rot, their rocBLAS handle; one plan per host thread at a time (thread-compatible,
+ *    not thread-safe); different plans may be used from different threads.
  */
 #ifndef REFSPI_AMD_H
 #define REFSPI_AMD_H
@@ -162,7 +163,8 @@ int rsp_maxlp_destroy(rsp_maxlp* plan);
  *   mode 0: FP32 in/out through rocBLAS sgemm (the plain FP32 form)
  *   mode 1: FP32 in/out, split-BF16 hi/lo products with FP32 accumulation on MFMA
  *   mode 2: FP64 in/out through rocBLAS dgemm
- * Mode 1 allocates its padded bf16 planes on `stream` for the call. */
+ * Mode 1 allocates its padded bf16 planes on `stream` for the call; modes 0 and 2 use a
+ * rocBLAS handle created for the call (so concurrent callers never share one). */
 int rsp_xterm_gemm(int mode, int64_t M, int64_t N, int64_t K, const void* A, const void* B,
                    void* C, rsp_stream_t stream);
 

@@ -868,6 +868,8 @@ struct rsp_rot {
   int64_t pass_max = 0, pc_alloc = 0;
   bool pass_first = true;
   double* d_Pc = nullptr;
+  // rocBLAS handle of this plan (FP32 / FP64 forms and the adjoint's FP64 products)
+  rocblas_handle blas = nullptr;
   // timing
   bool timing = false;
   hipEvent_t ev[8] = {};
@@ -1073,10 +1075,10 @@ int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, cons
     if (rc) return rc;
     if (p->timing) RSP_HIP(hipEventRecord(p->ev[3], st));
     if (p->mode == kFP32) {
-      rc = xterm_gemm_blas32((const float*)p->d_A, (const float*)p->d_B, (float*)p->d_X, p->Mpad,
+      rc = xterm_gemm_blas32(p->blas, (const float*)p->d_A, (const float*)p->d_B, (float*)p->d_X, p->Mpad,
                              n, p->Kp, p->Kp, p->Kp, p->Mpad, st);
     } else if (p->mode == kFP64) {
-      rc = xterm_gemm_blas64((const double*)p->d_A, (const double*)p->d_B, (double*)p->d_X,
+      rc = xterm_gemm_blas64(p->blas, (const double*)p->d_A, (const double*)p->d_B, (double*)p->d_X,
                              p->Mpad, n, p->Kp, p->Kp, p->Kp, p->Mpad, st);
     } else {
       const __bf16* ah = (const __bf16*)p->d_A;
@@ -1211,7 +1213,7 @@ int run_adj_frames(rsp_rot* p, const Csr& f, const double* prob, const double* s
                          st, prob, scales, d0, n, p->H, p->d_aQ);
       RSP_LAUNCH_CHECK("k_scaled_prob");
       // GT (H x Kp, column-major) (+)= Q^T B : GT[k * H + h] = sum_d Q[d, h] B[d, k]
-      if ((rc = xterm_dgemm(false, true, p->H, p->Kp, n, 1., p->d_aQ, p->H, p->d_aB, p->Kp,
+      if ((rc = xterm_dgemm(p->blas, false, true, p->H, p->Kp, n, 1., p->d_aQ, p->H, p->d_aB, p->Kp,
                             (gt_first && d0 == 0) ? 0. : 1., p->d_aGT, p->H, st)))
         return rc;
     }
@@ -1226,7 +1228,7 @@ int run_adj_frames(rsp_rot* p, const Csr& f, const double* prob, const double* s
                                    reinterpret_cast<float*>(p->d_aObar), p->fg_Mp, n, p->fg_Hp,
                                    p->fg_Mp, 0, st)))
           return rc;
-      } else if ((rc = xterm_dgemm(false, false, p->Kp, n, p->H, 1., p->d_aA, p->Kp,
+      } else if ((rc = xterm_dgemm(p->blas, false, false, p->Kp, n, p->H, 1., p->d_aA, p->Kp,
                                    prob + d0 * p->H, p->H, 0., p->d_aObar, p->Kp, st))) {
         // Obar (n x Kp, row-major) = P A : Obar[d * Kp + k] = sum_h P[d, h] A[h, k]
         return rc;
@@ -1423,6 +1425,11 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
   if (sizeof(double) * (size_t)(p->nrad + 1) + 2 * sizeof(double2) * (size_t)size > 60000) {
     rsp_rot_destroy(p);
     return fail(kErrSize, "rsp_rot_create: radius table exceeds LDS");
+  }
+  if (rocblas_create_handle(&p->blas) != rocblas_status_success) {
+    p->blas = nullptr;
+    rsp_rot_destroy(p);
+    return fail(kErrState, "rsp_rot_create: rocblas_create_handle failed");
   }
   *out = p;
   return kOk;
@@ -1742,6 +1749,7 @@ int rsp_rot_destroy(rsp_rot* p) {
   dfree(p->d_pk);
   for (hipEvent_t e : p->ev)
     if (e) (void)hipEventDestroy(e);
+  if (p->blas) rocblas_destroy_handle(p->blas);
   delete p;
   return kOk;
 }

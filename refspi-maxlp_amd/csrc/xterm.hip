@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "refspi_amd.h"
@@ -297,25 +298,6 @@ __global__ void k_slice_sum(const float* __restrict__ P, int64_t stride, int S, 
   C[i] = (float)s;
 }
 
-rocblas_handle g_blas = nullptr;
-int g_blas_dev = -1;
-
-int blas_handle(hipStream_t stream, rocblas_handle* out) {
-  int dev = 0;
-  RSP_HIP(hipGetDevice(&dev));
-  if (g_blas == nullptr || g_blas_dev != dev) {
-    if (g_blas) rocblas_destroy_handle(g_blas);
-    g_blas = nullptr;
-    if (rocblas_create_handle(&g_blas) != rocblas_status_success)
-      return rsp::fail(rsp::kErrState, "rocblas_create_handle failed");
-    g_blas_dev = dev;
-  }
-  if (rocblas_set_stream(g_blas, stream) != rocblas_status_success)
-    return rsp::fail(rsp::kErrState, "rocblas_set_stream failed");
-  *out = g_blas;
-  return rsp::kOk;
-}
-
 }  // namespace
 
 namespace rsp {
@@ -364,11 +346,11 @@ int xterm_gemm_split(const __bf16* ah, const __bf16* al, const __bf16* bh, const
 }
 
 // FP32 / FP64 through rocBLAS: C[d * ldc + r] = sum_k A[r * lda + k] B[d * ldb + k].
-int xterm_gemm_blas32(const float* A, const float* B, float* C, int64_t M, int64_t N,
-                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream) {
-  rocblas_handle h;
-  int rc = blas_handle(stream, &h);
-  if (rc) return rc;
+int xterm_gemm_blas32(rocblas_handle h, const float* A, const float* B, float* C, int64_t M,
+                      int64_t N, int64_t K, int64_t lda, int64_t ldb, int64_t ldc,
+                      hipStream_t stream) {
+  if (rocblas_set_stream(h, stream) != rocblas_status_success)
+    return fail(kErrState, "rocblas_set_stream failed");
   const float one = 1.f, zero = 0.f;
   rocblas_status s = rocblas_sgemm(h, rocblas_operation_transpose, rocblas_operation_none,
                                    (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &one, A,
@@ -378,11 +360,11 @@ int xterm_gemm_blas32(const float* A, const float* B, float* C, int64_t M, int64
   return kOk;
 }
 
-int xterm_gemm_blas64(const double* A, const double* B, double* C, int64_t M, int64_t N,
-                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream) {
-  rocblas_handle h;
-  int rc = blas_handle(stream, &h);
-  if (rc) return rc;
+int xterm_gemm_blas64(rocblas_handle h, const double* A, const double* B, double* C,
+                      int64_t M, int64_t N, int64_t K, int64_t lda, int64_t ldb, int64_t ldc,
+                      hipStream_t stream) {
+  if (rocblas_set_stream(h, stream) != rocblas_status_success)
+    return fail(kErrState, "rocblas_set_stream failed");
   const double one = 1., zero = 0.;
   rocblas_status s = rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none,
                                    (rocblas_int)M, (rocblas_int)N, (rocblas_int)K, &one, A,
@@ -392,16 +374,15 @@ int xterm_gemm_blas64(const double* A, const double* B, double* C, int64_t M, in
   return kOk;
 }
 
-int xterm_dgemm(bool transA, bool transB, int64_t m, int64_t n, int64_t k, double alpha,
-                const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                double* C, int64_t ldc, hipStream_t stream) {
+int xterm_dgemm(rocblas_handle h, bool transA, bool transB, int64_t m, int64_t n, int64_t k,
+                double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                double beta, double* C, int64_t ldc, hipStream_t stream) {
   if (m > 0x7fffffff || n > 0x7fffffff || k > 0x7fffffff || lda > 0x7fffffff ||
       ldb > 0x7fffffff || ldc > 0x7fffffff)
     return fail(kErrSize, "dgemm size too large");
   if (m == 0 || n == 0) return kOk;
-  rocblas_handle h;
-  int rc = blas_handle(stream, &h);
-  if (rc) return rc;
+  if (rocblas_set_stream(h, stream) != rocblas_status_success)
+    return fail(kErrState, "rocblas_set_stream failed");
   rocblas_status s = rocblas_dgemm(
       h, transA ? rocblas_operation_transpose : rocblas_operation_none,
       transB ? rocblas_operation_transpose : rocblas_operation_none, (rocblas_int)m,
@@ -413,46 +394,83 @@ int xterm_dgemm(bool transA, bool transB, int64_t m, int64_t n, int64_t k, doubl
 
 }  // namespace rsp
 
+namespace {
+
+// Split-BF16 GEMM of FP32 row-major A [M x K], B [N x K] into C [N x M] with the bf16 planes
+// (2 * (Mp + Np) * Kp elements) and the K-slice partial products (S * N * Mp floats) in
+// caller-provided device memory; every element of both is written before it is read.
+int split_gemm_into(int64_t M, int64_t N, int64_t K, const float* A, const float* B, float* C,
+                    __bf16* planes, float* Cp, hipStream_t stream) {
+  using namespace rsp;
+  const int64_t Mp = xterm_pad_m(M), Np = xterm_pad_n(N), Kp = xterm_pad_k(K);
+  __bf16 *ah = planes, *al = ah + Mp * Kp, *bh = al + Mp * Kp, *bl = bh + Np * Kp;
+  const int S = xterm_split_k(Kp);
+  int rc = xterm_split(A, M, K, K, ah, al, Mp, Kp, stream);
+  if (!rc) rc = xterm_split(B, N, K, K, bh, bl, Np, Kp, stream);
+  if (!rc) rc = xterm_gemm_split(ah, al, bh, bl, Cp, Mp, N, Kp, Mp, N * Mp, stream);
+  if (!rc) {
+    const int64_t n = N * M;
+    hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, Cp,
+                       N * Mp, S, M, Mp, n, C);
+    hipError_t ce = hipGetLastError();
+    if (ce != hipSuccess) rc = hip_fail(ce, "k_slice_sum");
+  }
+  return rc;
+}
+
+// Bytes of plane and partial-product memory split_gemm_into needs.
+void split_gemm_bytes(int64_t M, int64_t N, int64_t K, size_t* plane_bytes, size_t* part_bytes) {
+  using namespace rsp;
+  const int64_t Mp = xterm_pad_m(M), Np = xterm_pad_n(N), Kp = xterm_pad_k(K);
+  *plane_bytes = (size_t)(2 * Mp + 2 * Np) * Kp * sizeof(__bf16);
+  *part_bytes = (size_t)xterm_split_k(Kp) * N * Mp * sizeof(float);
+}
+
+int gemm_args(int64_t M, int64_t N, int64_t K, const void* A, const void* B, void* C) {
+  using namespace rsp;
+  if (M < 0 || N < 0 || K < 0) return fail(kErrSize, "negative size");
+  if ((M && N && K) && (!A || !B || !C)) return fail(kErrNull, "null buffer");
+  if (M > 0x7fffffff || N > 0x7fffffff || K > 0x7fffffff) return fail(kErrSize, "size too large");
+  return kOk;
+}
+
+}  // namespace
+
 extern "C" {
 
 int rsp_xterm_gemm(int mode, int64_t M, int64_t N, int64_t K, const void* A, const void* B,
                    void* C, rsp_stream_t stream_) {
   using namespace rsp;
   hipStream_t stream = (hipStream_t)stream_;
-  if (M < 0 || N < 0 || K < 0) return fail(kErrSize, "negative size");
-  if ((M && N && K) && (!A || !B || !C)) return fail(kErrNull, "null buffer");
-  if (M > 0x7fffffff || N > 0x7fffffff || K > 0x7fffffff) return fail(kErrSize, "size too large");
+  int rc = gemm_args(M, N, K, A, B, C);
+  if (rc) return rc;
   if (M == 0 || N == 0) return kOk;
-  if (mode == 0)
-    return xterm_gemm_blas32((const float*)A, (const float*)B, (float*)C, M, N, K, K, K, M,
+  if (mode == 0 || mode == 2) {
+    // a handle of the call's own: concurrent callers never share one
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success)
+      return fail(kErrState, "rocblas_create_handle failed");
+    if (mode == 0)
+      rc = xterm_gemm_blas32(h, (const float*)A, (const float*)B, (float*)C, M, N, K, K, K, M,
                              stream);
-  if (mode == 2)
-    return xterm_gemm_blas64((const double*)A, (const double*)B, (double*)C, M, N, K, K, K, M,
-                             stream);
+    else
+      rc = xterm_gemm_blas64(h, (const double*)A, (const double*)B, (double*)C, M, N, K, K, K,
+                             M, stream);
+    rocblas_destroy_handle(h);
+    return rc;
+  }
   if (mode != 1) return fail(kErrUnsupported, "mode must be 0 (FP32), 1 (split BF16), 2 (FP64)");
-  const int64_t Mp = xterm_pad_m(M), Np = xterm_pad_n(N), Kp = xterm_pad_k(K);
+  size_t plane_bytes = 0, part_bytes = 0;
+  split_gemm_bytes(M, N, K, &plane_bytes, &part_bytes);
   __bf16* planes = nullptr;
   float* Cp = nullptr;
-  const size_t nplane = (size_t)(2 * Mp + 2 * Np) * Kp;
-  RSP_HIP(hipMallocAsync((void**)&planes, nplane * sizeof(__bf16), stream));
-  hipError_t e = hipMallocAsync((void**)&Cp,
-                                (size_t)xterm_split_k(Kp) * N * Mp * sizeof(float), stream);
+  RSP_HIP(hipMallocAsync((void**)&planes, plane_bytes, stream));
+  hipError_t e = hipMallocAsync((void**)&Cp, part_bytes, stream);
   if (e != hipSuccess) {
     (void)hipFreeAsync(planes, stream);
     return hip_fail(e, "hipMallocAsync");
   }
-  __bf16 *ah = planes, *al = ah + Mp * Kp, *bh = al + Mp * Kp, *bl = bh + Np * Kp;
-  const int S = xterm_split_k(Kp);
-  int rc = xterm_split((const float*)A, M, K, K, ah, al, Mp, Kp, stream);
-  if (!rc) rc = xterm_split((const float*)B, N, K, K, bh, bl, Np, Kp, stream);
-  if (!rc) rc = xterm_gemm_split(ah, al, bh, bl, Cp, Mp, N, Kp, Mp, N * Mp, stream);
-  if (!rc) {
-    const int64_t n = N * M;
-    hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, Cp,
-                       N * Mp, S, M, Mp, n, (float*)C);
-    hipError_t ce = hipGetLastError();
-    if (ce != hipSuccess) rc = hip_fail(ce, "k_slice_sum");
-  }
+  rc = split_gemm_into(M, N, K, (const float*)A, (const float*)B, (float*)C, planes, Cp, stream);
   (void)hipFreeAsync(Cp, stream);
   (void)hipFreeAsync(planes, stream);
   return rc;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <cstdint>
 
@@ -31,16 +32,19 @@ int xterm_gemm_split(const __bf16* ah, const __bf16* al, const __bf16* bh, const
                      float* C, int64_t Mpad, int64_t N, int64_t Kp, int64_t ldc,
                      int64_t slice_stride, hipStream_t stream);
 
-// The same through rocBLAS in FP32 / FP64 (row-major A [M x lda], B [N x ldb]).
-int xterm_gemm_blas32(const float* A, const float* B, float* C, int64_t M, int64_t N,
-                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream);
-int xterm_gemm_blas64(const double* A, const double* B, double* C, int64_t M, int64_t N,
-                      int64_t K, int64_t lda, int64_t ldb, int64_t ldc, hipStream_t stream);
+// The same through rocBLAS in FP32 / FP64 (row-major A [M x lda], B [N x ldb]) on the
+// caller's handle (set to `stream` for the call; a handle is never shared between plans).
+int xterm_gemm_blas32(rocblas_handle h, const float* A, const float* B, float* C, int64_t M,
+                      int64_t N, int64_t K, int64_t lda, int64_t ldb, int64_t ldc,
+                      hipStream_t stream);
+int xterm_gemm_blas64(rocblas_handle h, const double* A, const double* B, double* C,
+                      int64_t M, int64_t N, int64_t K, int64_t lda, int64_t ldb, int64_t ldc,
+                      hipStream_t stream);
 
 // General FP64 GEMM in rocBLAS's column-major convention:
 //   C (m x n, ldc) = alpha * op(A) op(B) + beta * C, op = transpose when trans* is true.
-int xterm_dgemm(bool transA, bool transB, int64_t m, int64_t n, int64_t k, double alpha,
-                const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                double* C, int64_t ldc, hipStream_t stream);
+int xterm_dgemm(rocblas_handle h, bool transA, bool transB, int64_t m, int64_t n, int64_t k,
+                double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                double beta, double* C, int64_t ldc, hipStream_t stream);
 
 }  // namespace rsp
