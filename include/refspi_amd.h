@@ -168,6 +168,18 @@ int rsp_maxlp_destroy(rsp_maxlp* plan);
 int rsp_xterm_gemm(int mode, int64_t M, int64_t N, int64_t K, const void* A, const void* B,
                    void* C, rsp_stream_t stream);
 
+/* The split-BF16 form of rsp_xterm_gemm (mode 1) with no allocation: its bf16 planes and
+ * K-slice partial products live in a static device array of 32 MiB per GPU (held from the
+ * library's load until the process ends) instead of memory allocated per call.  No
+ * counterpart in the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Limit:
+ * (2*pad256(M) + 2*pad256(N)) * pad32(K) * 2 + ceil(pad32(K)/12288) * N * pad256(M) * 4
+ * bytes <= 33554432, else kErrSize (-2) before any device call.  Calls on one GPU share
+ * the array: calls that overlap in time on the same GPU (two streams, two threads) race;
+ * calls queued on one stream do not.  Results equal mode 1 of rsp_xterm_gemm bit for bit. */
+int rsp_xterm_gemm_static(int64_t M, int64_t N, int64_t K, const float* A, const float* B,
+                          float* C, rsp_stream_t stream);
+
 /* Rotation-form plan over the scored pixels (probmask == 0 of emc.py:89-98), with
  * hypotheses h = c*R + r (class c < C, set by option 3; rotation r < R = num_rot):
  *   O_h = bilinear rotation of class c's complex model by the r-th angle (kernels.cu:6-42

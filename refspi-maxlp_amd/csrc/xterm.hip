@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -434,6 +435,34 @@ int gemm_args(int64_t M, int64_t N, int64_t K, const void* A, const void* B, voi
   return kOk;
 }
 
+// Scratch of rsp_xterm_gemm_static: the bf16 planes and partial products of one call.  Each
+// GPU holds its own copy from the library's load until the process ends; its device address
+// comes from hipGetSymbolAddress (the array's name is a host-side stand-in in host code).
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+__device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_ptr(void** out) {
+  using namespace rsp;
+  int dev = 0;
+  RSP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(kErrUnsupported, "device index above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(kErrState, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
 }  // namespace
 
 extern "C" {
@@ -474,6 +503,26 @@ int rsp_xterm_gemm(int mode, int64_t M, int64_t N, int64_t K, const void* A, con
   (void)hipFreeAsync(Cp, stream);
   (void)hipFreeAsync(planes, stream);
   return rc;
+}
+
+int rsp_xterm_gemm_static(int64_t M, int64_t N, int64_t K, const float* A, const float* B,
+                          float* C, rsp_stream_t stream_) {
+  using namespace rsp;
+  hipStream_t stream = (hipStream_t)stream_;
+  int rc = gemm_args(M, N, K, A, B, C);
+  if (rc) return rc;
+  if (M == 0 || N == 0) return kOk;
+  size_t plane_bytes = 0, part_bytes = 0;
+  split_gemm_bytes(M, N, K, &plane_bytes, &part_bytes);
+  if (plane_bytes + part_bytes > kStaticScratchBytes)
+    return fail(kErrSize, "rsp_xterm_gemm_static: needs " +
+                              std::to_string(plane_bytes + part_bytes) +
+                              " bytes of scratch, more than its 33554432");
+  void* base = nullptr;
+  if ((rc = scratch_ptr(&base))) return rc;
+  __bf16* planes = (__bf16*)base;
+  float* Cp = (float*)((unsigned char*)base + plane_bytes);
+  return split_gemm_into(M, N, K, A, B, C, planes, Cp, stream);
 }
 
 }  // extern "C"

@@ -43,6 +43,7 @@ _SIGS = {
     'rsp_maxlp_info': (C.c_int, [vp, i64p]),
     'rsp_maxlp_destroy': (C.c_int, [vp]),
     'rsp_xterm_gemm': (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int64, vp, vp, vp, vp]),
+    'rsp_xterm_gemm_static': (C.c_int, [C.c_int64, C.c_int64, C.c_int64, vp, vp, vp, vp]),
     'rsp_rot_create': (C.c_int, [C.c_int, C.c_int64, C.c_int64, vp, vp, C.c_double, vp]),
     'rsp_rot_set_frames': (C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp]),
     'rsp_rot_set_option': (C.c_int, [vp, C.c_int, C.c_int64]),
