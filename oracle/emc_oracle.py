@@ -400,16 +400,19 @@ def photon_index(t, size, rc, rs, qx, qy):
 
     rots = [[c, -s], [s, c]] with c = cos(-a), s = sin(-a); the product is evaluated as
     ``c*qx + (-s)*qy`` and ``s*qx + c*qy`` without fused multiply-add, then
-    ``*size + size//2`` and rint (half to even).  The reference's CuPy matmul may fuse
-    these (its ulp behaviour at exact half-integers is parity-unpinned).  Indices that
-    reach ``size`` (even sizes only) are clamped to ``size-1`` as kernels.cu:280-281
-    does; the reference leaves that case undefined.
+    ``*size + size//2``, rint (half to even) and ``astype('i4')``; the flat index is
+    ``x*size + y`` with no clamp (maxLP.py:252).  At even sizes x or y can reach ``size``
+    for pixels near |q| = 0.5: y = size is then the first pixel of row x+1 (plain
+    arithmetic); x = size gives an index >= npix, which the photons matrix's fancy
+    indexing reduces modulo its npix columns (cupyx/scipy/sparse/_index.py
+    ``_asindices``: ``x % length``).  That last case is parity unpinned (CuPy is not
+    importable here), and so is the ulp behaviour of CuPy's matmul at exact half-integers.
     """
     px = (rc * qx[t] + (-rs) * qy[t]) * size + size // 2
     py = (rs * qx[t] + rc * qy[t]) * size + size // 2
-    x = np.clip(np.rint(px).astype('i4'), 0, size - 1)
-    y = np.clip(np.rint(py).astype('i4'), 0, size - 1)
-    return x * size + y
+    x = np.rint(px).astype('i4').astype(np.int64)
+    y = np.rint(py).astype('i4').astype(np.int64)
+    return ((x * size + y) % (size * size)).astype(np.int64)
 
 
 def fref_d(t, qx, qy, qrad, diams, shifts):

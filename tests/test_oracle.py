@@ -251,3 +251,104 @@ def test_pattern_search_break_returns_pre_step_value():
             break
         f, step = fn, sn
     assert out == f
+
+
+def test_photon_index_even_size_wraps_as_the_reference_indexes():
+    """maxLP.py:251-252 at an even size: rint can give x or y = size (no clamp).
+    size 16, centre 8.  Pixel (11, 15) has q*size = (3, 7), |q| = 0.476 < 0.5 (in mask).
+    With rots = [[c, -s], [s, c]], (c, s) = (7, 3)/|(7, 3)|: y' = 3s + 7c + 8 = 15.62 -> 16,
+    x' = 3c - 7s + 8 = 8 -> flat index 8*16 + 16 = 144, the first pixel of row 9.
+    Pixel (15, 11), (c, s) = (7, -3)/|(7, -3)|: x' = 7c - 3(-s)... = 15.62 -> 16,
+    y' = 7s + 3c + 8 = 8 -> 16*16 + 8 = 264 >= 256, which the photons matrix's fancy
+    indexing reduces modulo 256 (cupyx _asindices) to 8 = pixel (0, 8)."""
+    size = 16
+    qx, qy, _, mask, _ = O.qvals(size)
+    n = np.hypot(7., 3.)
+    t1 = 11 * size + 15
+    assert mask[t1]
+    got = O.photon_index(t1, size, np.array([7. / n]), np.array([3. / n]), qx, qy)
+    assert got[0] == 9 * size + 0
+    t2 = 15 * size + 11
+    assert mask[t2]
+    got = O.photon_index(t2, size, np.array([7. / n]), np.array([-3. / n]), qx, qy)
+    assert got[0] == 8
+    # odd sizes never reach `size`: every index stays the plain rint of the rotation
+    s = 17
+    qx, qy, _, mask, _ = O.qvals(s)
+    ang = np.linspace(0, np.pi, 181)
+    rc, rs = O.frame_rot_cos_sin(ang)
+    for t in np.nonzero(mask)[0]:
+        px = np.rint((rc * qx[t] - rs * qy[t]) * s + s // 2)
+        py = np.rint((rs * qx[t] + rc * qy[t]) * s + s // 2)
+        assert px.max() <= s - 1 and py.max() <= s - 1
+
+
+def test_merge_all_equals_dense_product():
+    """kernels.cu:139-152: view[p] = sum_d prob_r[d] * K_dp, i.e. K^T prob_r (brute force
+    on the densified frames)."""
+    fr, K = _frames(n=25)
+    rng = np.random.default_rng(11)
+    pr = rng.random(fr.num_data)
+    view = O.merge_all(pr, fr.ones, fr.multi, fr.ones_accum, fr.multi_accum, fr.place_ones,
+                       fr.place_multi, fr.count_multi, 17 * 17)
+    np.testing.assert_allclose(view, K.T @ pr, rtol=1e-13, atol=1e-15)
+    assert view[np.nonzero(K.sum(0) == 0)[0]].max(initial=0.) == 0.
+
+
+def test_rotation_angles_are_a_half_turn():
+    """emc.py:192 and 223: rotation r is the angle r*pi/R.  A frame whose photons follow the
+    log-view rotated by pi/2 (built with slice_gen at the explicit angle) must take the
+    rotation r = R/2 of R = 8 (a table of r*2*pi/R would put pi/2 at r = 2), and its
+    unravelled angle is pi/2."""
+    size = 21
+    rng = np.random.default_rng(12)
+    bg = np.zeros(size * size)
+    model = 0.2 * (rng.random(size * size) + 1j * rng.random(size * size))
+    states = O.state_grid((0.5, 0.5, 1), (-0.5, -0.5, 1), (8., 8., 1))
+    sx, sy, dia, ns = states
+    holo = O.slice_gen_holo(model, sx[0], sy[0], dia[0], 1., 1., size, bg, 0)
+    rot = O.slice_gen(holo, np.pi / 2, 1., size, bg, 0)
+    pm = O.probmask(size)
+    lam = np.where(pm < 1, rot, 0.)
+    K = np.floor(400. * lam / lam.max()).astype(int)
+    po = np.repeat(np.arange(size * size), K).astype('i4')
+    fr = O.SparseFramesNP([len(po)], [0], po, np.zeros(0, 'i4'), np.zeros(0, 'i4'),
+                          size * size)
+    prob, _, _ = O.estep_prob(model, size, (sx, sy, dia), 8, fr)
+    _, rmax = O.get_rmax(prob)
+    assert rmax[0] == 4
+    ang = O.unravel_rmax(rmax, ns, 8, sx, sy, dia)[3]
+    assert ang[0] == np.pi / 2
+
+
+def test_run_phaser_composition_single_frame():
+    """maxLP.py:304-340 with one frame, where every quantity has a closed form:
+    * pixels outside `mask` are returned unchanged;
+    * phase 2's rescale is the mean over the rmask pixels of K_t / |f_t + R_t|^2 at the
+      phase-1 values f_t (get_rescale_pixel with N = 1);
+    * phase 3 maximises K log(r|f+R|^2) - r|f+R|^2 with that global r, whose optimum is the
+      circle r|f + R|^2 = K: every searched pixel with K > 0 ends close to it."""
+    size = 17
+    rng = np.random.default_rng(13)
+    K = rng.integers(1, 6, size * size)
+    po = np.repeat(np.arange(size * size), K).astype('i4')
+    fr = O.SparseFramesNP([len(po)], [0], po, np.zeros(0, 'i4'), np.zeros(0, 'i4'),
+                          size * size)
+    model = 0.3 * (rng.random(size * size) + 1j * rng.random(size * size)) + 0.2
+    orc = O.MaxLPOracle(fr, size)
+    args = (np.array([0.5]), np.array([-0.3]), np.array([8.]), np.array([0.]))
+    ring = np.nonzero(orc.rmask)[0]
+    after1 = orc.run_phaser(model, *args, pixels=[])        # phase 3 over no pixel
+    out = orc.run_phaser(model, *args)
+    np.testing.assert_array_equal(out[~orc.mask], model[~orc.mask])
+    np.testing.assert_array_equal(after1[~orc.rmask], model[~orc.rmask])
+    Rt = np.array([O.fref_d(t, orc.qx, orc.qy, orc.qrad, np.array([8.]),
+                            np.array([[0.5, -0.3]]))[0] for t in range(size * size)])
+    # angle 0: the detector pixel of model pixel t is t itself, so K_t is the frame's count
+    resc = np.mean(K[ring] / np.abs(after1[ring] + Rt[ring]) ** 2)
+    assert abs(orc.last_rescale - resc) <= 1e-13 * resc
+    m = np.nonzero(orc.mask)[0]
+    w = orc.last_rescale * np.abs(out[m] + Rt[m]) ** 2
+    # the search stops when its step is below 1e-3 of |f|: within a few steps of the circle
+    assert np.median(np.abs(w / K[m] - 1.)) < 2e-4
+    assert np.max(np.abs(w / K[m] - 1.)) < 3e-3
