@@ -59,6 +59,26 @@ int rsp_calc_prob_all(const double* lview, const int32_t* mask, int64_t ndata,
 int rsp_get_f_dt(const double* fobj_t, const double* fref_d, int64_t ndata, int64_t npix,
                  double* f_dt, rsp_stream_t stream);
 
+/* kernels.cu:84-114 slice_merge: model[m] += sum view[t]*wx*wy and mweights[m] += sum
+ * wx*wy over the view pixels t whose bilinear sample under `angle` (cos/sin on the host)
+ * touches m.  Gather form (one thread per model pixel, no atomics): each pixel's terms are
+ * summed in view-pixel order and then added, so the result differs from the reference's
+ * atomic scatter only by the association of the additions. */
+int rsp_slice_merge(const double* view, double angle, int64_t size, double* model,
+                    double* mweights, rsp_stream_t stream);
+
+/* kernels.cu:139-152 merge_all: view[p] += prob_r[d] for every single photon of frame d at
+ * p and prob_r[d]*count for every multi photon.  npix (not in the reference's list) bounds
+ * the photon pixels: any pixel outside [0, npix) fails the call with -2 before view is
+ * touched.  No atomics: the photons are sorted by (pixel, frame-major position) and each
+ * pixel's terms are added to view[p] one after another in frame order, ones before multi --
+ * the order of a sequential loop over the reference's threads.  Allocates its sort buffers
+ * on `stream`; synchronises `stream`. */
+int rsp_merge_all(const double* prob_r, int64_t ndata, const int32_t* ones,
+                  const int32_t* multi, const int64_t* o_acc, const int64_t* m_acc,
+                  const int32_t* p_o, const int32_t* p_m, const int32_t* c_m, double* view,
+                  int64_t npix, rsp_stream_t stream);
+
 /* ---------------------------------------------------------------------------------
  * EMC E-step plan: replaces emc.py:159-216 (_calculate_prob + _get_rmax) for one GPU.
  * Hypotheses are h = s*num_rot + r (s = state index of the raveled 'ij' meshgrid of
@@ -147,8 +167,25 @@ int rsp_maxlp_run(rsp_maxlp* plan, double* model_c, const double* sx, const doub
                   const double* dia, const double* rot_cos, const double* rot_sin,
                   double* rescale_out, rsp_stream_t stream);
 
-/* Sizes of the plan (host int64[5]): frames, photon entries, model-space photon entries
- * of the last run, ring pixels, mask pixels. */
+/* maxLP.py:245-253 get_photons_pixel for all frames: K_d(t) (device f64 N) of model pixel t
+ * in the frames and rotations of the last rsp_maxlp_run, read from the model-space photon
+ * lists that run pulled back (zero where frame d has no photon at t's detector pixel). */
+int rsp_maxlp_photons_pixel(const rsp_maxlp* plan, int64_t t, double* k_out,
+                            rsp_stream_t stream);
+
+/* maxLP.py:181-197 get_logq_pixel: LL_c = mean_d K_d log w_dc - mean_d w_dc for ncand
+ * (1..1280) candidates cand (device c16) at mask pixel t, with the frames and per-frame
+ * parameters of the last rsp_maxlp_run; rescale < 0 takes each candidate's
+ * get_rescale_pixel (maxLP.py:216-228).  out: device f64 ncand.  The same arithmetic as
+ * the pattern search of rsp_maxlp_run (product form of the log term, moments for the mean). */
+int rsp_maxlp_logq_pixel(const rsp_maxlp* plan, int64_t t, const double* cand, int64_t ncand,
+                         double rescale, double* out, rsp_stream_t stream);
+
+/* Sizes and work of the plan (host int64[8]): [0] frames, [1] photon entries,
+ * [2] model-space photon entries of the last run, [3] ring pixels, [4] mask pixels;
+ * of the last run: [5] entry x candidate products evaluated by the pattern search (each
+ * one |c + R_d|^2 factor: 6 FP64 operations), [6] search steps evaluated (over all pixels
+ * of both phases), [7] microseconds of the two pattern-search launches (HIP events). */
 int rsp_maxlp_info(const rsp_maxlp* plan, int64_t* out);
 
 int rsp_maxlp_destroy(rsp_maxlp* plan);

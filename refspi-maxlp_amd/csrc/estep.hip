@@ -1314,9 +1314,14 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
     return fail(kErrNull, "rsp_emc_set_frames: null photon list");
   for (int64_t i = 0; i < no; ++i)
     if (place_ones[i] < 0 || place_ones[i] >= e->npix) return fail(kErrSize, "place_ones out of range");
-  for (int64_t i = 0; i < nm; ++i)
+  for (int64_t i = 0; i < nm; ++i) {
     if (place_multi[i] < 0 || place_multi[i] >= e->npix)
       return fail(kErrSize, "place_multi out of range");
+    // a multi photon's count is at least 2 in the reference's files; below 1 would shrink
+    // the exactness bound of the FP32 pass, and the slot lists replicate each count
+    if (count_multi[i] < 1 || count_multi[i] >= (1 << 22))
+      return fail(kErrSize, "count_multi out of range (1 <= count < 2^22)");
+  }
   e->N = N;
   // background (dset.py:56-60) and powder (dset.py:74-83)
   std::vector<double> hbg(e->npix, 0.);

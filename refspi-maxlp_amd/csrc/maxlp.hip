@@ -54,15 +54,18 @@ struct Geo {
   const double* qrad;
 };
 
-// maxLP.py:250-252 (+ clamp of kernels.cu:280-281 for even sizes; see the oracle).
+// maxLP.py:250-252: x, y = rint(rots @ q * size + size//2), flat index x*size + y with no
+// clamp.  At even sizes x or y reaches `size` for pixels near |q| = 0.5: y = size is the
+// first pixel of row x+1; x = size (index >= npix) wraps modulo npix, as the photons
+// matrix's fancy indexing reduces column indices (cupyx _asindices: x % length).
 __device__ __forceinline__ int detector_index(const Geo& g, int t, double c, double s) {
   const double qx = g.qx[t], qy = g.qy[t];
   const double px = (c * qx + (-s) * qy) * g.size + (double)(g.size / 2);
   const double py = (s * qx + c * qy) * g.size + (double)(g.size / 2);
-  int x = (int)rint(px), y = (int)rint(py);
-  x = min(max(x, 0), g.size - 1);
-  y = min(max(y, 0), g.size - 1);
-  return x * g.size + y;
+  const int x = (int)rint(px), y = (int)rint(py);
+  int idx = (x * g.size + y) % g.npix;
+  if (idx < 0) idx += g.npix;
+  return idx;
 }
 
 // maxLP.py:238-243 get_fref_d for one (pixel, frame)
@@ -98,23 +101,48 @@ __global__ __launch_bounds__(256) void k_pull(Geo g, const int32_t* __restrict__
   const int d = fr[e], p = pix[e];
   const double c = rc[d], s = rs[d];
   const int c0 = g.size / 2;
-  const double X = (double)(p / g.size - c0), Y = (double)(p % g.size - c0);
-  // inverse of [[c, -s], [s, c]]
-  const double u = c * X + s * Y + c0, v = -s * X + c * Y + c0;
-  const int u0 = (int)rint(u), v0 = (int)rint(v);
+  const int px = p / g.size, py = p % g.size;
   int n = 0;
   int64_t w = mode ? pos[e] : 0;
-  for (int du = -2; du <= 2; ++du)
-    for (int dv = -2; dv <= 2; ++dv) {
-      const int uu = u0 + du, vv = v0 + dv;
-      if (uu < 0 || uu >= g.size || vv < 0 || vv >= g.size) continue;
-      const int t = uu * g.size + vv;
-      if (!mask[t]) continue;
-      if (detector_index(g, t, c, s) != p) continue;
-      if (mode)
-        keys[w++] = ((uint64_t)t << 47) | ((uint64_t)d << 22) | (uint64_t)cnt[e];
-      ++n;
+  // Model pixels whose detector index is p lie within 0.71 px of the inverse rotation of
+  // an unreduced index (X, Y) with X*size + Y == p (mod npix): (px, py) itself, and at the
+  // wrap-around of detector_index (even sizes) (px-1, size) for py == 0 and (size, py) for
+  // px == 0.  Each is searched in a 5x5 window, skipping the cells of earlier windows (so
+  // no model pixel is found twice), and every candidate is checked with the forward index.
+  int u0[3], v0[3], nw = 0;
+  for (int k = 0; k < 3; ++k) {
+    int X, Y;
+    if (k == 0) {
+      X = px; Y = py;
+    } else if (k == 1) {
+      if (py != 0 || px < 1) continue;
+      X = px - 1; Y = g.size;
+    } else {
+      if (px != 0) continue;
+      X = g.size; Y = py;
     }
+    const double Xc = (double)(X - c0), Yc = (double)(Y - c0);
+    // inverse of [[c, -s], [s, c]]
+    const double u = c * Xc + s * Yc + c0, v = -s * Xc + c * Yc + c0;
+    u0[nw] = (int)rint(u);
+    v0[nw] = (int)rint(v);
+    ++nw;
+  }
+  for (int k = 0; k < nw; ++k)
+    for (int du = -2; du <= 2; ++du)
+      for (int dv = -2; dv <= 2; ++dv) {
+        const int uu = u0[k] + du, vv = v0[k] + dv;
+        if (uu < 0 || uu >= g.size || vv < 0 || vv >= g.size) continue;
+        bool earlier = false;
+        for (int j = 0; j < k; ++j) earlier |= (abs(uu - u0[j]) <= 2 && abs(vv - v0[j]) <= 2);
+        if (earlier) continue;
+        const int t = uu * g.size + vv;
+        if (!mask[t]) continue;
+        if (detector_index(g, t, c, s) != p) continue;
+        if (mode)
+          keys[w++] = ((uint64_t)t << 47) | ((uint64_t)d << 22) | (uint64_t)cnt[e];
+        ++n;
+      }
   if (!mode) nmatch[e] = n;
 }
 
@@ -145,35 +173,50 @@ __global__ __launch_bounds__(256) void k_entries(Geo g, const uint64_t* __restri
   ent[i] = e;
 }
 
-// Per-pixel moments m1 = mean_d R_d(t), m2 = mean_d |R_d(t)|^2 and K-bar = sum K / N.
+// Per-pixel moments m1 = mean_d R_d(t) and v = mean_d |R_d(t) - m1|^2 (second pass over the
+// frames, so that mean_d |f + R_d|^2 = |f + m1|^2 + v is a sum of two non-negative terms
+// with no cancellation, whatever f).  Fixed reduction order (deterministic).
 __global__ __launch_bounds__(256) void k_moments(Geo g, const int32_t* __restrict__ pixels,
                                                  int N, const double* __restrict__ sx,
                                                  const double* __restrict__ sy,
                                                  const double* __restrict__ dia,
                                                  double* __restrict__ mom) {
-  __shared__ double red[3][256];
+  __shared__ double red[2][256];
   const int t = pixels[blockIdx.x];
-  double a = 0., b = 0., q = 0.;
+  double a = 0., b = 0.;
   for (int d = threadIdx.x; d < N; d += 256) {
     double rr, ri;
     fref(g, t, dia[d], sx[d], sy[d], rr, ri);
     a += rr;
     b += ri;
-    q += rr * rr + ri * ri;
   }
   red[0][threadIdx.x] = a;
   red[1][threadIdx.x] = b;
-  red[2][threadIdx.x] = q;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w)
-      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+      for (int k = 0; k < 2; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  const double m1r = red[0][0] / N, m1i = red[1][0] / N;
+  __syncthreads();
+  double q = 0.;
+  for (int d = threadIdx.x; d < N; d += 256) {
+    double rr, ri;
+    fref(g, t, dia[d], sx[d], sy[d], rr, ri);
+    const double er = rr - m1r, ei = ri - m1i;
+    q += er * er + ei * ei;
+  }
+  red[0][threadIdx.x] = q;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[0][threadIdx.x] += red[0][threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    mom[4 * t + 0] = red[0][0] / N;
-    mom[4 * t + 1] = red[1][0] / N;
-    mom[4 * t + 2] = red[2][0] / N;
+    mom[4 * t + 0] = m1r;
+    mom[4 * t + 1] = m1i;
+    mom[4 * t + 2] = red[0][0] / N;
   }
 }
 
@@ -199,33 +242,63 @@ __device__ __forceinline__ bool cand_better(double a, int ia, double b, int ib) 
   return a > b || (a == b && ia < ib);
 }
 
+// prod * a^K as m * 2^xs without leaving FP64's normal range for any K and a: a = m_a 2^e
+// contributes K*e exactly and m_a^K (m_a in [0.5, 1)) is formed with a renormalisation
+// every 8 factors.
+__device__ __forceinline__ double pow_renorm(double prod, double a, int K, int* xs) {
+  int ea;
+  const double ma = frexp(a, &ea);
+  double pr = prod;
+  int x = ea * K;
+  for (int k = 0; k < K; ++k) {
+    pr = pr * ma;
+    if ((k & 7) == 7) {
+      int e;
+      pr = frexp(pr, &e);
+      x += e;
+    }
+  }
+  *xs = x;
+  return pr;
+}
+
+// mean_d |c + R_d|^2 from the moments (no cancellation: see k_moments)
+__device__ __forceinline__ double mean_w(double cr, double ci, double m1r, double m1i, double v) {
+  const double ar = cr + m1r, ai = ci + m1i;
+  return (ar * ar + ai * ai) + v;
+}
+
 // One workgroup per pixel: up to 10 pattern-search steps (maxLP.py:154-179).
 // rescale < 0 selects the per-candidate rescale of phase 1 (maxLP.py:187).  Work-group b
 // takes list position i = low 32 bits of order[b] (k_order_keys, sorted); the pixel's
 // model value and resc_out[i] are its only outputs, so the order changes no result.
-__global__ __launch_bounds__(kThreads) void k_pattern(
+// kProbe: one evaluation of the caller's candidates probe_cand[0..probe_n) at pixel probe_t
+// (get_logq_pixel, maxLP.py:181-197) written to probe_out; same arithmetic as the search.
+template <bool kProbe>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pattern(
     const uint64_t* __restrict__ order, const int32_t* __restrict__ pixels,
     const int64_t* __restrict__ seg_beg,
     const int64_t* __restrict__ seg_end, const Entry* __restrict__ ent,
     const double* __restrict__ mom, int N, double rescale, double* __restrict__ model,
-    double* __restrict__ resc_out) {
+    double* __restrict__ resc_out, int probe_t, const double* __restrict__ probe_cand,
+    int probe_n, double* __restrict__ probe_out, unsigned long long* __restrict__ counters) {
   __shared__ Entry sh[kChunk];
   __shared__ double wv[kThreads / 64];
   __shared__ int wi[kThreads / 64];
   __shared__ double skt;
-  const int li = (int)(uint32_t)order[blockIdx.x];
-  const int t = pixels[li];
+  const int li = kProbe ? 0 : (int)(uint32_t)order[blockIdx.x];
+  const int t = kProbe ? probe_t : pixels[li];
   const int64_t e0 = seg_beg[t], e1 = seg_end[t];
   // issue priority by work: the heaviest pixels (dispatched first) would otherwise share
   // their SIMDs evenly with the light ones that fill the CU around them and finish last;
   // with priority the arbiter issues their waves first and the light waves fill the gaps
-  {
+  if (!kProbe) {
     const int64_t n = e1 - e0;
     if (n >= 16384) __builtin_amdgcn_s_setprio(3);
     else if (n >= 4096) __builtin_amdgcn_s_setprio(2);
     else if (n >= 1024) __builtin_amdgcn_s_setprio(1);
   }
-  const double m1r = mom[4 * t + 0], m1i = mom[4 * t + 1], m2 = mom[4 * t + 2];
+  const double m1r = mom[4 * t + 0], m1i = mom[4 * t + 1], vr = mom[4 * t + 2];
   // K-bar = mean_d K_d (exact integer sum)
   {
     double ks = 0.;
@@ -241,17 +314,25 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
     __syncthreads();
   }
   const double kbar = skt;
-  double fr = model[2 * t], fi = model[2 * t + 1];
+  double fr = kProbe ? 0. : model[2 * t], fi = kProbe ? 0. : model[2 * t + 1];
   double step = hypot(fr, fi) / 4.;
-  for (int it = 0; it < 10; ++it) {
+  int rounds = 0;  // candidate evaluation rounds (search steps evaluated)
+  for (int it = 0; it < (kProbe ? 1 : 10); ++it) {
+    ++rounds;
     double cr[kPerThread], ci[kPerThread], prod[kPerThread];
     int ex[kPerThread];
 #pragma unroll
     for (int u = 0; u < kPerThread; ++u) {
       const int c = threadIdx.x + kThreads * u;
-      const double px = -8. + 0.5 * (double)(c / 33), py = -8. + 0.5 * (double)(c % 33);
-      cr[u] = fr + px * step;
-      ci[u] = fi + py * step;
+      if (kProbe) {
+        const int cc = c < probe_n ? c : 0;
+        cr[u] = probe_cand[2 * cc];
+        ci[u] = probe_cand[2 * cc + 1];
+      } else {
+        const double px = -8. + 0.5 * (double)(c / 33), py = -8. + 0.5 * (double)(c % 33);
+        cr[u] = fr + px * step;
+        ci[u] = fi + py * step;
+      }
       prod[u] = 1.;
       ex[u] = 0;
     }
@@ -273,25 +354,37 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
             const double re = cr[u] + E.rr, im = ci[u] + E.ri;
             prod[u] = prod[u] * (re * re + im * im);
           }
+          // renormalise every 8 factors (scaling by powers of two is exact, so the period
+          // does not change the product's rounding while 8 factors stay within FP64's
+          // normal range: |f_c + R_d|^2 in [2^-120, 2^120]; an exact zero stays zero)
+          if ((i & 7) == 7) {
+#pragma unroll
+            for (int u = 0; u < kPerThread; ++u) {
+              int e;
+              prod[u] = frexp(prod[u], &e);
+              ex[u] += e;
+            }
+          }
         } else {
+          // a^K, then one renormalisation.  With K <= 8 and a in [2^-120, 2^120] the K
+          // products stay within FP64's normal range (prod is in [0.5, 1) here); otherwise
+          // a = m 2^e (m in [0.5, 1)) contributes K*e to the exponent exactly and m^K is
+          // formed with a renormalisation every 8 factors.  Scaling by powers of two is
+          // exact, so both forms give the same bits wherever the first stays normal.
 #pragma unroll
           for (int u = 0; u < kPerThread; ++u) {
             const double re = cr[u] + E.rr, im = ci[u] + E.ri;
             const double a = re * re + im * im;
             double pr = prod[u] * a;
-            for (int k = 1; k < K; ++k) pr = pr * a;
-            prod[u] = pr;
-          }
-        }
-        // renormalise every 8 factors (scaling by powers of two is exact, so the period
-        // does not change the product's rounding while 8 factors stay within FP64's
-        // normal range: |f_c + R_d|^2 in [1e-38, 1e38])
-        if ((i & 7) == 7 || K > 1) {
-#pragma unroll
-          for (int u = 0; u < kPerThread; ++u) {
+            int xs = 0;
+            if (K <= 8 && a >= 0x1p-120 && a <= 0x1p120) {
+              for (int k = 1; k < K; ++k) pr = pr * a;
+            } else {
+              pr = pow_renorm(prod[u], a, K, &xs);
+            }
             int e;
-            prod[u] = frexp(prod[u], &e);
-            ex[u] += e;
+            prod[u] = frexp(pr, &e);
+            ex[u] += xs + e;
           }
         }
       }
@@ -302,16 +395,18 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
 #pragma unroll
     for (int u = 0; u < kPerThread; ++u) {
       const int c = threadIdx.x + kThreads * u;
-      if (c >= kCand) continue;
-      const double mc = cr[u] * cr[u] + ci[u] * ci[u] + 2. * (cr[u] * m1r + ci[u] * m1i) + m2;
+      if (c >= (kProbe ? probe_n : kCand)) continue;
+      const double mc = mean_w(cr[u], ci[u], m1r, m1i, vr);
       const double slog = (log(prod[u]) + (double)ex[u] * M_LN2) / N;
       const double r = rescale < 0. ? kbar / mc : rescale;
       const double v = kbar * log(r) + slog - r * mc;
+      if (kProbe) probe_out[c] = v;
       if (cand_better(v, c, bv, bi)) {
         bv = v;
         bi = c;
       }
     }
+    if (kProbe) return;
     for (int off = 32; off > 0; off >>= 1) {
       const double ov = __shfl_xor(bv, off);
       const int oi = __shfl_xor(bi, off);
@@ -342,14 +437,36 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
     step = ns;
   }
   if (threadIdx.x == 0) {
+    // work counters: entry x candidate products and search steps evaluated
+    atomicAdd(&counters[0], (unsigned long long)rounds * (unsigned long long)(e1 - e0) *
+                                (unsigned long long)kCand);
+    atomicAdd(&counters[1], (unsigned long long)rounds);
     model[2 * t] = fr;
     model[2 * t + 1] = fi;
     if (resc_out) {
       // maxLP.py:216-228 get_rescale_pixel at the final value
-      const double mc = fr * fr + fi * fi + 2. * (fr * m1r + fi * m1i) + m2;
-      resc_out[li] = kbar / mc;
+      resc_out[li] = kbar / mean_w(fr, fi, m1r, m1i, vr);
     }
   }
+}
+
+// get_photons_pixel (maxLP.py:245-253) from the pulled-back entries of pixel t: K_d(t) for
+// every frame (duplicate detector photons of a frame add up, as the photons CSR sums them).
+__global__ __launch_bounds__(256) void k_photons_pixel(int t, const int64_t* __restrict__ seg_beg,
+                                                       const int64_t* __restrict__ seg_end,
+                                                       const uint64_t* __restrict__ keys,
+                                                       double* __restrict__ k_out) {
+  const int64_t e0 = seg_beg[t], e1 = seg_end[t];
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+    const uint64_t k = keys[e];
+    const int d = (int)((k >> 22) & ((1u << 25) - 1));
+    atomicAdd(&k_out[d], (double)(k & ((1u << 22) - 1)));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ x, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = v;
 }
 
 }  // namespace rsp
@@ -372,6 +489,13 @@ struct rsp_maxlp {
   size_t tmp_cap = 0;
   double last_rescale = 0.;
   int64_t n_entries = 0;
+  bool ran = false;
+  // work counters of the last run (entry x candidate products, search steps) and the
+  // k_pattern time of both phases (HIP events on the run's stream)
+  unsigned long long* d_counters = nullptr;
+  unsigned long long counters[2] = {0, 0};
+  hipEvent_t ev[4] = {};
+  double pattern_ms = 0.;
 };
 
 namespace {
@@ -440,9 +564,18 @@ extern "C" int rsp_maxlp_create(int device, int64_t size, rsp_maxlp** out) {
       (rc = alloc(&m->d_seg_end, m->npix)) ||
       (rc = alloc(&m->d_resc, std::max<int64_t>(1, (int64_t)m->rmask_pix.size()))) ||
       (rc = alloc(&m->d_ord, (int64_t)std::max(m->rmask_pix.size(), m->mask_pix.size()))) ||
-      (rc = alloc(&m->d_ord2, (int64_t)std::max(m->rmask_pix.size(), m->mask_pix.size())))) {
-    delete m;
+      (rc = alloc(&m->d_ord2, (int64_t)std::max(m->rmask_pix.size(), m->mask_pix.size()))) ||
+      (rc = alloc(&m->d_counters, 2))) {
+    rsp_maxlp_destroy(m);
     return rc;
+  }
+  for (hipEvent_t& e : m->ev) {
+    const hipError_t he = hipEventCreate(&e);
+    if (he != hipSuccess) {
+      e = nullptr;
+      rsp_maxlp_destroy(m);
+      return hip_fail(he, "hipEventCreate");
+    }
   }
   *out = m;
   return kOk;
@@ -484,6 +617,7 @@ extern "C" int rsp_maxlp_set_frames(rsp_maxlp* m, int64_t num_data, const int32_
     return rc;
   m->N = num_data;
   m->nph = (int64_t)fr.size();
+  m->ran = false;
   return kOk;
 }
 
@@ -526,6 +660,7 @@ extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, co
       return rc;
     m->key_cap = total;
   }
+  RSP_HIP(hipMemsetAsync(m->d_counters, 0, 2 * sizeof(unsigned long long), st));
   RSP_HIP(hipMemsetAsync(m->d_seg_beg, 0, sizeof(int64_t) * m->npix, st));
   RSP_HIP(hipMemsetAsync(m->d_seg_end, 0, sizeof(int64_t) * m->npix, st));
   if (total > 0) {
@@ -563,13 +698,16 @@ extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, co
     return kOk;
   };
   // phase 1: rmask annulus with per-candidate rescale (maxLP.py:317-320)
+  RSP_HIP(hipEventRecord(m->ev[0], st));
   if (nr > 0) {
     if ((rc = order(m->d_rmask_pix, nr))) return rc;
-    hipLaunchKernelGGL(k_pattern, dim3(nr), dim3(kThreads), 0, st, (const uint64_t*)m->d_ord2,
-                       m->d_rmask_pix, m->d_seg_beg, m->d_seg_end, m->d_ent, m->d_mom,
-                       (int)m->N, -1., model_c, m->d_resc);
+    hipLaunchKernelGGL(k_pattern<false>, dim3(nr), dim3(kThreads), 0, st,
+                       (const uint64_t*)m->d_ord2, m->d_rmask_pix, m->d_seg_beg, m->d_seg_end,
+                       m->d_ent, m->d_mom, (int)m->N, -1., model_c, m->d_resc, 0,
+                       (const double*)nullptr, 0, (double*)nullptr, m->d_counters);
     RSP_LAUNCH_CHECK("k_pattern(phase1)");
   }
+  RSP_HIP(hipEventRecord(m->ev[1], st));
   // phase 2: mean rescale over the annulus (maxLP.py:324-329)
   double rescale = 0.;
   if (nr > 0) {
@@ -608,14 +746,63 @@ extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, co
   m->last_rescale = rescale;
   if (rescale_out) *rescale_out = rescale;
   // phase 3: full mask with the global rescale (maxLP.py:332-336)
+  RSP_HIP(hipEventRecord(m->ev[2], st));
   if (nmask > 0) {
     if ((rc = order(m->d_mask_pix, nmask))) return rc;
-    hipLaunchKernelGGL(k_pattern, dim3(nmask), dim3(kThreads), 0, st, (const uint64_t*)m->d_ord2,
-                       m->d_mask_pix, m->d_seg_beg, m->d_seg_end, m->d_ent, m->d_mom,
-                       (int)m->N, rescale, model_c, (double*)nullptr);
+    hipLaunchKernelGGL(k_pattern<false>, dim3(nmask), dim3(kThreads), 0, st,
+                       (const uint64_t*)m->d_ord2, m->d_mask_pix, m->d_seg_beg, m->d_seg_end,
+                       m->d_ent, m->d_mom, (int)m->N, rescale, model_c, (double*)nullptr, 0,
+                       (const double*)nullptr, 0, (double*)nullptr, m->d_counters);
     RSP_LAUNCH_CHECK("k_pattern(phase3)");
   }
+  RSP_HIP(hipEventRecord(m->ev[3], st));
+  RSP_HIP(hipMemcpyAsync(m->counters, m->d_counters, 2 * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, st));
   RSP_HIP(hipStreamSynchronize(st));
+  // both phases' k_pattern launches (the dispatch-order sort between them is excluded)
+  float t1 = 0.f, t3 = 0.f;
+  RSP_HIP(hipEventElapsedTime(&t1, m->ev[0], m->ev[1]));
+  RSP_HIP(hipEventElapsedTime(&t3, m->ev[2], m->ev[3]));
+  m->pattern_ms = (double)t1 + (double)t3;
+  m->ran = true;
+  return kOk;
+}
+
+extern "C" int rsp_maxlp_photons_pixel(const rsp_maxlp* m, int64_t t, double* k_out,
+                                       rsp_stream_t stream_) {
+  if (!m || !k_out) return fail(kErrNull, "rsp_maxlp_photons_pixel: null pointer");
+  if (t < 0 || t >= m->npix) return fail(kErrSize, "rsp_maxlp_photons_pixel: pixel out of range");
+  if (!m->ran) return fail(kErrState, "rsp_maxlp_photons_pixel: no rsp_maxlp_run yet");
+  RSP_HIP(hipSetDevice(m->device));
+  hipStream_t st = (hipStream_t)stream_;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)ceil_div(m->N, 256)), dim3(256), 0, st, k_out, m->N,
+                     0.);
+  RSP_LAUNCH_CHECK("k_fill");
+  if (m->n_entries > 0) {
+    hipLaunchKernelGGL(k_photons_pixel, dim3(1), dim3(256), 0, st, (int)t, m->d_seg_beg,
+                       m->d_seg_end, m->d_keys2, k_out);
+    RSP_LAUNCH_CHECK("k_photons_pixel");
+  }
+  return kOk;
+}
+
+extern "C" int rsp_maxlp_logq_pixel(const rsp_maxlp* m, int64_t t, const double* cand,
+                                    int64_t ncand, double rescale, double* out,
+                                    rsp_stream_t stream_) {
+  if (!m || !cand || !out) return fail(kErrNull, "rsp_maxlp_logq_pixel: null pointer");
+  if (ncand < 1 || ncand > kThreads * kPerThread)
+    return fail(kErrSize, "rsp_maxlp_logq_pixel: 1 <= ncand <= 1280");
+  if (t < 0 || t >= m->npix) return fail(kErrSize, "rsp_maxlp_logq_pixel: pixel out of range");
+  if (!std::binary_search(m->mask_pix.begin(), m->mask_pix.end(), (int32_t)t))
+    return fail(kErrSize, "rsp_maxlp_logq_pixel: pixel outside the MaxLP mask");
+  if (!m->ran) return fail(kErrState, "rsp_maxlp_logq_pixel: no rsp_maxlp_run yet");
+  RSP_HIP(hipSetDevice(m->device));
+  hipStream_t st = (hipStream_t)stream_;
+  hipLaunchKernelGGL(k_pattern<true>, dim3(1), dim3(kThreads), 0, st, (const uint64_t*)nullptr,
+                     (const int32_t*)nullptr, m->d_seg_beg, m->d_seg_end, m->d_ent, m->d_mom,
+                     (int)m->N, rescale, (double*)nullptr, (double*)nullptr, (int)t, cand,
+                     (int)ncand, out, (unsigned long long*)nullptr);
+  RSP_LAUNCH_CHECK("k_pattern(probe)");
   return kOk;
 }
 
@@ -626,6 +813,9 @@ extern "C" int rsp_maxlp_info(const rsp_maxlp* m, int64_t* out) {
   out[2] = m->n_entries;
   out[3] = (int64_t)m->rmask_pix.size();
   out[4] = (int64_t)m->mask_pix.size();
+  out[5] = (int64_t)m->counters[0];
+  out[6] = (int64_t)m->counters[1];
+  out[7] = (int64_t)(m->pattern_ms * 1000. + 0.5);
   return kOk;
 }
 
@@ -638,6 +828,9 @@ extern "C" int rsp_maxlp_destroy(rsp_maxlp* m) {
                   m->d_ent, m->d_tmp};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  if (m->d_counters) hipFree(m->d_counters);
+  for (hipEvent_t e : m->ev)
+    if (e) hipEventDestroy(e);
   delete m;
   return kOk;
 }

@@ -433,34 +433,6 @@ __global__ __launch_bounds__(256) void k_merge_view(
   }
 }
 
-// Visit the scored view pixels j whose bilinear sample under rotation r touches model
-// pixel (mx, my), with the bilinear weight w: f(j, w).
-template <typename F>
-__device__ __forceinline__ void for_each_source(int size, int mx, int my, double ac, double as,
-                                                const int32_t* __restrict__ compact_of, F f) {
-  const int cen = size / 2;
-  // inverse rotation of m: the view pixels that can reach m lie within +-1 of it
-  const double mxc = (double)(mx - cen), myc = (double)(my - cen);
-  const int x0 = (int)rint(mxc * ac + myc * as + cen);
-  const int y0 = (int)rint(-mxc * as + myc * ac + cen);
-  for (int dx = -1; dx <= 1; ++dx)
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int x = x0 + dx, y = y0 + dy;
-      if (x < 0 || x >= size || y < 0 || y >= size) continue;
-      const int j = compact_of[x * size + y];
-      if (j < 0) continue;
-      double tx, ty;
-      rot_source(x, y, cen, ac, as, tx, ty);
-      const int ix = (int)floor(tx), iy = (int)floor(ty);
-      if (ix < 0 || ix > size - 2 || iy < 0 || iy > size - 2) continue;
-      const int ox = mx - ix, oy = my - iy;
-      if (ox < 0 || ox > 1 || oy < 0 || oy > 1) continue;
-      const double fx = tx - ix, fy = ty - iy;
-      const double cx = 1. - fx, cy = 1. - fy;
-      f(j, (ox ? fx : cx) * (oy ? fy : cy));
-    }
-}
-
 // slice_merge of every rotation of class c = blockIdx.y in gather form; one thread per
 // model pixel m.  Outputs at c * npix + m.
 __global__ void k_merge_model(int size, int R, int Hs, const double* __restrict__ rcos,
@@ -478,7 +450,8 @@ __global__ void k_merge_model(int size, int R, int Hs, const double* __restrict_
   for (int r = 0; r < R; ++r) {
     const int h = c * R + r;
     double wr = 0.;
-    for_each_source(size, mx, my, rcos[r], rsin[r], compact_of, [&](int j, double w) {
+    for_each_source(size, mx, my, rcos[r], rsin[r], compact_of, [&](int j, double wx, double wy) {
+      const double w = wx * wy;
       acc += VT[(int64_t)j * Hs + h] * w;
       wr += w;
     });
@@ -780,7 +753,8 @@ __global__ void k_adj_model(int size, int R, int Hs, const double* __restrict__ 
   double gr = 0., gi = 0.;
   for (int r = 0; r < R; ++r) {
     const int h = c * R + r;
-    for_each_source(size, mx, my, rcos[r], rsin[r], compact_of, [&](int j, double w) {
+    for_each_source(size, mx, my, rcos[r], rsin[r], compact_of, [&](int j, double wx, double wy) {
+      const double w = wx * wy;
       const double2 g = GO[(int64_t)j * Hs + h];
       gr += g.x * w;
       gi += g.y * w;
@@ -1327,6 +1301,8 @@ int pack_frames(const rsp_rot* p, int64_t N, const int32_t* ones, const int32_t*
     for (int q = 0; q < multi[d]; ++q, ++im) {
       const int t = place_multi[im];
       if (t < 0 || t >= p->npix) return fail(kErrSize, "photon index out of range");
+      if (count_multi[im] < 1 || count_multi[im] >= (1 << 22))
+        return fail(kErrSize, "count_multi out of range (1 <= count < 2^22)");
       const int j = p->compact_of[t];
       if (j >= 0) {
         fj.push_back(j);

@@ -94,4 +94,35 @@ __device__ __forceinline__ double holo_pixel(const double* __restrict__ model_c,
   return a * a;
 }
 
+// Visit the view pixels j whose bilinear sample (kernels.cu:19-33) under the rotation
+// (ac, as) touches model pixel (mx, my), with the bilinear weight's two factors wx * wy:
+// f(j, wx, wy) -- the gather
+// form of slice_merge's scatter (kernels.cu:84-114).  compact_of maps a pixel to its index
+// j (< 0: skipped); null visits every pixel with j = its flat index.
+template <typename F>
+__device__ __forceinline__ void for_each_source(int size, int mx, int my, double ac, double as,
+                                                const int32_t* __restrict__ compact_of, F f) {
+  const int cen = size / 2;
+  // inverse rotation of m: the view pixels that can reach m lie within +-1 of it
+  const double mxc = (double)(mx - cen), myc = (double)(my - cen);
+  const int x0 = (int)rint(mxc * ac + myc * as + cen);
+  const int y0 = (int)rint(-mxc * as + myc * ac + cen);
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int x = x0 + dx, y = y0 + dy;
+      if (x < 0 || x >= size || y < 0 || y >= size) continue;
+      const int j = compact_of ? compact_of[x * size + y] : x * size + y;
+      if (j < 0) continue;
+      double tx, ty;
+      rot_source(x, y, cen, ac, as, tx, ty);
+      const int ix = (int)floor(tx), iy = (int)floor(ty);
+      if (ix < 0 || ix > size - 2 || iy < 0 || iy > size - 2) continue;
+      const int ox = mx - ix, oy = my - iy;
+      if (ox < 0 || ox > 1 || oy < 0 || oy > 1) continue;
+      const double fx = tx - ix, fy = ty - iy;
+      const double cx = 1. - fx, cy = 1. - fy;
+      f(j, ox ? fx : cx, oy ? fy : cy);
+    }
+}
+
 }  // namespace rsp

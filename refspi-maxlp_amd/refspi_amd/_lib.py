@@ -28,6 +28,8 @@ _SIGS = {
     'rsp_calc_prob_all': (C.c_int, [vp, vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, C.c_double,
                                     vp, vp, vp]),
     'rsp_get_f_dt': (C.c_int, [vp, vp, C.c_int64, C.c_int64, vp, vp]),
+    'rsp_slice_merge': (C.c_int, [vp, C.c_double, C.c_int64, vp, vp, vp]),
+    'rsp_merge_all': (C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int64, vp]),
     'rsp_emc_create': (C.c_int, [C.c_int, C.c_int64, C.c_int64, f64p, f64p, f64p, f64p, f64p,
                                  C.c_int64, C.POINTER(vp)]),
     'rsp_emc_set_frames': (C.c_int, [vp, C.c_int64, i32p, i32p, i32p, i32p, i32p, f64p]),
@@ -40,6 +42,8 @@ _SIGS = {
     'rsp_maxlp_create': (C.c_int, [C.c_int, C.c_int64, C.POINTER(vp)]),
     'rsp_maxlp_set_frames': (C.c_int, [vp, C.c_int64, i32p, i32p, i32p, i32p, i32p]),
     'rsp_maxlp_run': (C.c_int, [vp, vp, vp, vp, vp, vp, vp, f64p, vp]),
+    'rsp_maxlp_photons_pixel': (C.c_int, [vp, C.c_int64, vp, vp]),
+    'rsp_maxlp_logq_pixel': (C.c_int, [vp, C.c_int64, vp, C.c_int64, C.c_double, vp, vp]),
     'rsp_maxlp_info': (C.c_int, [vp, i64p]),
     'rsp_maxlp_destroy': (C.c_int, [vp]),
     'rsp_xterm_gemm': (C.c_int, [C.c_int, C.c_int64, C.c_int64, C.c_int64, vp, vp, vp, vp]),

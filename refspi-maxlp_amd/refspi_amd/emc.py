@@ -137,6 +137,12 @@ class EMC:
         if self.dset.num_pix != self.size ** 2:
             raise AttributeError('Number of pixels in photons file does not match')
         self.powder = self.dset.get_powder()
+        # the generator's object, when the photons file holds it (make_data.py:164-167);
+        # used only for the FRC quality report of recon
+        self.solution = None
+        if dataset is None and self.photons_file is not None:
+            from . import h5io
+            self.solution = h5io.read_fields(self.photons_file, ['solution']).get('solution')
         etime = time.time()
         print('%d frames with %.3f photons/frame (%.3f s) (%.2f MB)' %
               (self.dset.num_data, self.dset.mean_count, etime - stime, self.dset.mem / 1024 ** 2))

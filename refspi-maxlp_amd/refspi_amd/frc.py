@@ -101,3 +101,19 @@ class FRC:
         self.best_angle = best[1]
         print('Best correlation for %.3f degrees' % best[1])
         return np.arange(rsize, dtype='f8') * binsize, best[2]
+
+
+def model_frc(model, solution, num_rot=360, radmin=4):
+    """Quality of a reconstruction: the FRC of ``model`` (the complex q-space model, flat
+    or square) against ``solution`` (real-space object or centred transform), with the best
+    of ``num_rot`` in-plane rotations (calc_frc.py:77-117).  Returns (mean FRC over rings
+    radmin .. size//2, per-ring FRC, best angle in degrees)."""
+    import contextlib
+    import io
+    sol = np.asarray(solution)
+    size = sol.shape[-1]
+    f = FRC(np.asarray(model).reshape(size, size), sol)
+    with contextlib.redirect_stdout(io.StringIO()):
+        _, cc = f.calc_rot(num_rot=num_rot, radmin=radmin)
+    cen = size // 2
+    return float(cc[radmin:cen].mean()), cc[:cen], float(f.best_angle)

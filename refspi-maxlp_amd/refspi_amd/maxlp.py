@@ -83,7 +83,9 @@ class MaxLPhaser:
         out = np.zeros(8, dtype='i8')
         _lib.call('rsp_maxlp_info', self._h, _lib.host_ptr(out, C.c_int64))
         return dict(frames=int(out[0]), photons=int(out[1]), model_entries=int(out[2]),
-                    ring_pixels=int(out[3]), mask_pixels=int(out[4]))
+                    ring_pixels=int(out[3]), mask_pixels=int(out[4]),
+                    entry_candidate_products=int(out[5]), search_steps=int(out[6]),
+                    pattern_ms=out[7] / 1000.)
 
     def run_device(self, model_dev, sx, sy, dia, ang, stream=None):
         """Update ``model_dev`` (torch complex128 on the GPU, npix) in place."""
@@ -103,6 +105,27 @@ class MaxLPhaser:
                   _lib.stream_handle(stream))
         self.rescale = resc.value
         return model_dev
+
+    def get_photons_pixel(self, t):
+        """maxLP.py:245-253 for all frames: K_d(t) (NumPy f8[N]) of model pixel t at the
+        rotations of the last run, read from the photon lists that run pulled back."""
+        import torch
+        out = torch.empty(self.num_data, dtype=torch.float64, device='cuda:%d' % self.device)
+        _lib.call('rsp_maxlp_photons_pixel', self._h, int(t), _lib.ptr(out), _lib.stream_handle())
+        return out.cpu().numpy()
+
+    def get_logq_pixel(self, cands, t, rescale=None):
+        """maxLP.py:181-197: LL of the candidate values (up to 1280) at mask pixel t with the
+        frames and per-frame parameters of the last run; rescale None = per candidate."""
+        import torch
+        dev = 'cuda:%d' % self.device
+        c = torch.as_tensor(np.ascontiguousarray(np.asarray(cands, dtype='c16').ravel()),
+                            device=dev)
+        out = torch.empty(c.numel(), dtype=torch.float64, device=dev)
+        _lib.call('rsp_maxlp_logq_pixel', self._h, int(t), _lib.ptr(c), c.numel(),
+                  -1. if rescale is None else float(rescale), _lib.ptr(out),
+                  _lib.stream_handle())
+        return out.cpu().numpy()
 
     def _run_phaser(self, model, sx_vals, sy_vals, dia_vals, ang_vals):
         """maxLP.py:304-340: returns the updated model as NumPy c16 (flat)."""

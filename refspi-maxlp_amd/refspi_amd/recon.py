@@ -5,7 +5,8 @@
 Runs ``NUM_ITER`` EMC + MaxLP iterations on one GPU, writes ``output_%03d.h5`` per
 iteration and ``EMC.log`` (header ``Iter  time(s)  change``, one line per iteration with
 the wall time and the L2 norm of the model change, recon.py:45-60) into the config's
-``output_folder``.  A devices file (one device number per line) selects the GPU of
+``output_folder``; when the photons file holds the object (``solution``), also ``FRC.log``
+with each iteration's FRC against it (calc_frc.py:77-117).  A devices file (one device number per line) selects the GPU of
 rank 0 (recon.py:30-40); there is one rank.
 """
 from __future__ import annotations
@@ -42,11 +43,18 @@ def log_line(it, seconds, change):
     return '%-6d%-.2e %e\n' % (it, seconds, change)
 
 
-def run(recon, num_iter, logf=None, out=sys.stdout):
-    """recon.py:43-63 loop; returns the per-iteration (time, change) list."""
+def run(recon, num_iter, logf=None, out=sys.stdout, frcf=None):
+    """recon.py:43-63 loop; returns the per-iteration (time, change) list.  With ``frcf``
+    and a ``solution`` on the reconstructor (the object stored in the photons file), each
+    iteration's FRC against it (calc_frc.py:77-117, best of 360 rotations, mean over rings
+    4 .. size//2) is written there as ``iteration FRC``."""
+    solution = getattr(recon, 'solution', None)
     if logf is not None:
         logf.write('Iter  time(s)  change\n')
         logf.flush()
+    if frcf is not None and solution is not None:
+        frcf.write('Iter  FRC\n')
+        frcf.flush()
     hist = []
     avgtime, numavg = 0., 0
     for i in range(num_iter):
@@ -61,6 +69,10 @@ def run(recon, num_iter, logf=None, out=sys.stdout):
             logf.flush()
         print('Change from last iteration: ', norm, file=out)
         hist.append((etime - stime, norm))
+        if frcf is not None and solution is not None:
+            from .frc import model_frc
+            frcf.write('%-6d%.6f\n' % (i + 1, model_frc(recon.model, solution)[0]))
+            frcf.flush()
         if i > 0:
             avgtime += etime - stime
             numavg += 1
@@ -77,7 +89,11 @@ def main(argv=None):
         print('Running on default device 0')
     recon = EMC(args.config_file, num_streams=args.streams, device=dev)
     with open(op.join(recon.output_folder, 'EMC.log'), 'w') as logf:
-        run(recon, args.num_iter, logf)
+        if recon.solution is not None:
+            with open(op.join(recon.output_folder, 'FRC.log'), 'w') as frcf:
+                run(recon, args.num_iter, logf, frcf=frcf)
+        else:
+            run(recon, args.num_iter, logf)
 
 
 if __name__ == '__main__':

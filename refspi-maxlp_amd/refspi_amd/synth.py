@@ -312,15 +312,25 @@ def model_scales(cfg: SynthConfig, cls: int = 0):
     return float(alpha[cls]), rel
 
 
+def solution(cfg: SynthConfig, cls: int = 0):
+    """The generator's object as a real-space size x size image (the ``solution`` dataset
+    of make_data.py:164-167): the real part of the centred inverse transform of its scaled
+    q-space transform."""
+    f = (model_scales(cfg, cls)[0] * object_ft(cfg, cls)).reshape(cfg.size, cfg.size)
+    return np.real(np.fft.fftshift(np.fft.ifftn(np.fft.ifftshift(f))))
+
+
 def write_photons_file(path, cfg: SynthConfig):
     """Generate ``cfg`` with the NumPy back end and write a photons file in the reference
-    layout (make_data.py:228-275: sparse photons, ``num_pix``, ``scale``, ``true_*``)."""
+    layout (make_data.py:228-275: sparse photons, ``num_pix``, ``scale``, ``true_*``, and the
+    object as ``solution``, make_data.py:164-167)."""
     from . import h5io
     fr = make_frames(cfg)
     t = fr.truth
     h5io.write_photons(path, fr, extra={'scale': t['fluence'], 'true_shifts': t['shifts'],
                                         'true_diameters': t['diameters'],
-                                        'true_angles': t['angles']})
+                                        'true_angles': t['angles'],
+                                        'solution': solution(cfg)})
     return fr
 
 

@@ -66,3 +66,4 @@ def test_rotation_form_argument_errors_without_gpu():
     assert lib.rsp_rot_create(0, 64, 0, cs, cs, 1., C.byref(out)) == -2      # no rotations
     assert lib.rsp_rot_create(0, 64, 2000, cs, cs, 1., C.byref(out)) == -2   # > 1536
     assert lib.rsp_xterm_gemm(1, -1, 1, 1, None, None, None, None) == -2
+

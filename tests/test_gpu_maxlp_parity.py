@@ -1,0 +1,187 @@
+"""MaxLP parity beyond the small grid: the photon pull-back, the candidate log-likelihood at
+large N, and the even-size photon index (maxLP.py:181-197, 245-253, 304-340).
+
+* get_photons_pixel: K_d(t) of every (mask pixel, frame) as the GPU pulled the photons
+  back equals the oracle's direct index (rint, no clamp, flat index x*size + y reduced
+  modulo npix), bit for bit, at 64^2 (where the even-size wrap occurs) and 21^2.
+* get_logq_pixel at 185^2 and 100,000 frames (a bright pixel holds ~10^5 photon
+  entries): the GPU's product-form log term and moment-form mean agree with the oracle's
+  direct FP64 sums within the bound derived in ``_bound``.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+U = 2. ** -53
+
+
+def _dataset_from_torch(f):
+    from refspi_amd.dset import Dataset
+    from refspi_amd.synth import SparseFrames
+    h = {k: f[k].cpu().numpy() for k in ('ones', 'multi', 'place_ones', 'place_multi',
+                                         'count_multi')}
+    fr = SparseFrames(f['num_pix'], h['ones'], h['multi'], h['place_ones'], h['place_multi'],
+                      h['count_multi'])
+    return fr, Dataset(frames=fr)
+
+
+@pytest.mark.parametrize('size,num_rot', [(64, 90), (21, 12)])
+def test_photons_pixel_equals_oracle_index(size, num_rot):
+    from oracle import emc_oracle as O
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.maxlp import MaxLPhaser
+    cfg = synth.SynthConfig(size=size, num_data=200, mean_count=150.,
+                            n_blobs=40 if size == 64 else 8,
+                            disk_radius=12. if size == 64 else 4., beamstop=3., seed=11)
+    fr = synth.make_frames(cfg)
+    # the generator leaves the detector's edge rows empty (|q| >= 0.5 is outside its
+    # mask); the even-size wrap lands exactly there (first pixel of a row, first row), so
+    # every frame gets one photon on each pixel of row 0 and column 0
+    edge = np.unique(np.concatenate([np.arange(size), np.arange(size) * size])).astype('i4')
+    po = np.split(fr.place_ones, np.cumsum(fr.ones)[:-1])
+    fr.place_ones = np.concatenate([np.concatenate([p, edge]) for p in po]).astype('i4')
+    fr.ones = (fr.ones + len(edge)).astype('i4')
+    ofr = O.SparseFramesNP(fr.ones, fr.multi, fr.place_ones, fr.place_multi, fr.count_multi,
+                           size * size)
+    rng = np.random.default_rng(1)
+    n = fr.num_data
+    ang = rng.integers(0, num_rot, n) * np.pi / num_rot
+    sx, sy = rng.uniform(-4, 4, n), rng.uniform(-4, 4, n)
+    dia = rng.uniform(7, 9, n)
+    ph = MaxLPhaser(Dataset(frames=fr), size=size)
+    model = 0.05 * (rng.random(size * size) + 1j * rng.random(size * size))
+    ph._run_phaser(model, sx, sy, dia, ang)
+    qx, qy, _, mask, _ = O.qvals(size)
+    rc, rs = O.frame_rot_cos_sin(ang)
+    dense = ofr.dense()
+    rows = np.arange(n)
+    wraps = 0
+    for t in np.nonzero(mask)[0]:
+        idx = O.photon_index(t, size, rc, rs, qx, qy)
+        want = dense[rows, idx]
+        got = ph.get_photons_pixel(t)
+        assert np.array_equal(got, want), (t, np.nonzero(got != want)[0][:5])
+        px = np.rint((rc * qx[t] - rs * qy[t]) * size + size // 2)
+        py = np.rint((rs * qx[t] + rc * qy[t]) * size + size // 2)
+        wraps += int((((px == size) | (py == size)) & (want > 0)).sum())
+    if size % 2 == 0:
+        assert wraps > 0, 'no photon reached through the even-size wrap'
+
+
+def _bound(k, R, cands, rescale, N):
+    """Bound on |LL_gpu - LL_oracle| per candidate c, from first-order rounding analysis:
+    w_dc = r|z_dc|^2, z_dc = c + R_d.  Each side evaluates R_d to within 4u|R_d| (sin, cos,
+    the cube), so w_dc carries a relative error e_dc <= 16u (1 + |R_d|/|z_dc|) on either
+    side (roundings of the sums, squares, rescale and that of R_d).  The log term then
+    differs by at most sum_d K_d e_dc (product form: one rounding per factor, included) plus
+    the oracle's sequential summation, gamma_N sum_d |K_d log w_dc|; the mean term by
+    sum_d w_dc e_dc plus gamma_N sum_d w_dc.  The per-candidate rescale enters only to
+    second order (dLL/dr = 0 at r = mean K / mean |z|^2)."""
+    z = cands[None, :] + R[:, None]
+    az = np.abs(z)
+    r = rescale if rescale is not None else k.mean() / (az ** 2).mean(0)
+    w = r * az ** 2
+    e = 16 * U * (1. + np.abs(R)[:, None] / az)
+    gam = N * U / (1. - N * U)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        klog = np.where(k[:, None] > 0, k[:, None] * np.abs(np.log(w)), 0.)
+    s = (k[:, None] * e).sum(0) + gam * klog.sum(0) + (w * e).sum(0) + gam * w.sum(0)
+    return s / N + 16 * U * (np.abs(k.mean() * np.log(r)) + (w.mean(0)))
+
+
+def test_logq_pixel_at_100k_frames_within_bound():
+    import torch
+    from oracle import c_oracle as CO
+    from oracle import emc_oracle as O
+    from refspi_amd import synth
+    from refspi_amd.maxlp import MaxLPhaser
+    cfg = synth.SynthConfig(**{**synth.CONFIG_185.__dict__, 'seed': 21})
+    size, N = cfg.size, cfg.num_data
+    f = synth.make_frames_torch(cfg)
+    fr, ds = _dataset_from_torch(f)
+    ang = (f['angles'].cpu().numpy() % np.pi)
+    sh = f['shifts'].cpu().numpy()
+    dia = f['diameters'].cpu().numpy()
+    del f
+    torch.cuda.empty_cache()
+    model = synth.model_scales(cfg)[0] * synth.object_ft(cfg)
+    ph = MaxLPhaser(ds, size=size)
+    out = ph._run_phaser(model, sh[:, 0], sh[:, 1], dia, ang)
+    ofr = O.SparseFramesNP(fr.ones, fr.multi, fr.place_ones, fr.place_multi, fr.count_multi,
+                           size * size)
+    mc = CO.MaxLPC(ofr, size)
+    mc.set_frames(sh[:, 0], sh[:, 1], dia, ang)
+    qrad = mc.qrad
+    mask_pix = np.nonzero(mc.mask)[0]
+    # the brightest pixels (smallest |q| in the mask), one at mid radius, a ring pixel and
+    # the outermost (no photons: outside the detector)
+    order = mask_pix[np.argsort(qrad[mask_pix], kind='stable')]
+    ring = np.nonzero(mc.rmask)[0]
+    pixels = [int(order[0]), int(order[3]), int(order[len(order) // 3]), int(ring[0]),
+              int(order[-1])]
+    pattern = O.pattern(8)
+    worst = 0.
+    for t in pixels:
+        f0 = out[t]
+        for rescale in (None, ph.rescale):
+            cands = f0 + pattern * (abs(f0) / 400.)
+            got = ph.get_logq_pixel(cands, t, rescale)
+            ref, k, R = mc.logq(t, cands, rescale)
+            assert np.array_equal(k, ph.get_photons_pixel(t))
+            # a pixel without photons has rescale 0 in phase 1: 0 * log 0 = NaN for every
+            # candidate on both sides (the reference's argmax then takes the first NaN)
+            np.testing.assert_array_equal(np.isnan(got), np.isnan(ref))
+            if np.isnan(ref).all():
+                assert k.sum() == 0 and rescale is None
+                continue
+            B = _bound(k, R, cands, rescale, N)
+            ratio = np.abs(got - ref) / B
+            worst = max(worst, float(ratio.max()))
+            assert np.all(ratio <= 1.), (t, rescale, ratio.max(), k.sum())
+            # the candidate the search would take agrees unless the oracle's top two are
+            # closer than twice the bound
+            i, j = int(np.argmax(got)), int(np.argmax(ref))
+            if i != j:
+                assert abs(ref[i] - ref[j]) <= 2 * max(B[i], B[j])
+    print('\nlargest |LL_gpu - LL_oracle| / bound at N = %d: %.3g' % (N, worst))
+
+
+@pytest.mark.parametrize('count', [5, 100, 4000])
+def test_logq_pixel_high_counts_near_cancellation(count):
+    """Entries with K > 8 and factors |c + R_d|^2 far below 1: the product form takes K*e
+    of a = m 2^e into the exponent exactly (a^K alone would leave FP64's range, e.g.
+    (1e-4)^100), and agrees with the oracle's direct sum of K log w within the bound."""
+    from oracle import c_oracle as CO
+    from oracle import emc_oracle as O
+    from refspi_amd.dset import Dataset
+    from refspi_amd.maxlp import MaxLPhaser
+    from refspi_amd.synth import SparseFrames
+    size, n = 21, 64
+    qx, qy, qrad, mask, _ = O.qvals(size)
+    t = int(np.nonzero(mask)[0][30])
+    # angle 0: the detector pixel of model pixel t is t; every frame holds `count` photons
+    # there and one elsewhere
+    i4 = lambda a: np.asarray(a, dtype='i4')
+    fr = SparseFrames(size * size, i4(np.ones(n)), i4(np.ones(n)), i4(np.full(n, 3)),
+                      i4(np.full(n, t)), i4(np.full(n, count)))
+    sx, sy = np.zeros(n), np.zeros(n)
+    dia = np.full(n, 8.)
+    ang = np.zeros(n)
+    ph = MaxLPhaser(Dataset(frames=fr), size=size)
+    model = np.full(size * size, 0.01 + 0.02j)
+    ph._run_phaser(model, sx, sy, dia, ang)
+    assert np.array_equal(ph.get_photons_pixel(t), np.full(n, float(count)))
+    R = O.fref_d(t, qx, qy, qrad, dia[:1], np.zeros((1, 2)))[0]
+    cands = -R + 1e-2 * abs(R) * (O.pattern(8) + (0.3 + 0.2j))
+    ofr = O.SparseFramesNP(fr.ones, fr.multi, fr.place_ones, fr.place_multi, fr.count_multi,
+                           size * size)
+    mc = CO.MaxLPC(ofr, size)
+    mc.set_frames(sx, sy, dia, ang)
+    for rescale in (None, 2.5):
+        got = ph.get_logq_pixel(cands, t, rescale)
+        ref, k, Rd = mc.logq(t, cands, rescale)
+        assert np.all(np.isfinite(got)) and np.all(np.isfinite(ref))
+        B = _bound(k, Rd, cands, rescale, n)
+        assert np.all(np.abs(got - ref) <= B), (np.abs(got - ref) / B).max()
