@@ -10,8 +10,9 @@ Metric: frame x rotation x state log-likelihood evaluations per second of whole
 iterations, N * R * S / t_iteration.
 
 Prints one JSON line with the metric, the live roofline of the dominant kernel (the
-sparse log-likelihood gather) and a CPU baseline (the NumPy FP64 oracle timed on a
-bounded sample of the same workload on this host).
+sparse log-likelihood gather, LDS-bound) and of MaxLP's pattern search (FP64 VALU), the FRC
+of the reconstruction against the generator's object, and a CPU baseline (the oracle's C
+restatement timed on a bounded sample of the same workload on this host, extrapolated).
 """
 import argparse
 import json
@@ -51,26 +52,135 @@ def make_dataset(args):
     return cfg, fr
 
 
-def cpu_baseline(fr, size, num_rot, grid, n_frames, n_states):
-    """NumPy FP64 oracle on a bounded sample: n_frames x n_states x num_rot hypotheses."""
+def host_facts():
+    """The host the CPU baseline ran on."""
+    model = 'unknown'
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(cpu_model=model, affinity_cores=len(os.sched_getaffinity(0)),
+                cpu_count=os.cpu_count(),
+                env={k: os.environ.get(k) for k in ('OMP_NUM_THREADS', 'OPENBLAS_NUM_THREADS',
+                                                    'MKL_NUM_THREADS')})
+
+
+def _sub_frames(fr, n):
     from oracle import emc_oracle as O
-    no = int(fr.ones[:n_frames].sum())
-    nm = int(fr.multi[:n_frames].sum())
-    ofr = O.SparseFramesNP(fr.ones[:n_frames], fr.multi[:n_frames], fr.place_ones[:no],
-                           fr.place_multi[:nm], fr.count_multi[:nm], fr.num_pix)
-    sx, sy, dia, _ = O.state_grid(*grid)
-    sel = np.linspace(0, len(sx) - 1, n_states).astype(int)
+    no = int(fr.ones[:n].sum())
+    nm = int(fr.multi[:n].sum())
+    return O.SparseFramesNP(fr.ones[:n], fr.multi[:n], fr.place_ones[:no], fr.place_multi[:nm],
+                            fr.count_multi[:nm], fr.num_pix)
+
+
+def cpu_baseline(fr, size, num_rot, grid, n_frames, n_states, n_pixels):
+    """The oracle's C restatement (FP64, the reference's operation order, OpenMP threads) on
+    a bounded sample of the same workload: the E-step of n_frames frames x n_states states
+    (spread over the grid) x num_rot rotations, and MaxLP phase 1 / phase 3 on n_pixels
+    ring / mask pixels of those frames.  A whole iteration of the full workload is
+    extrapolated linearly (E-step in frames x states, MaxLP in frames x pixels)."""
+    from oracle import c_oracle as CO
+    from oracle import emc_oracle as O
+    CO.load()
+    nth = CO.threads()
+    N = fr.ones.shape[0]
+    ofr = _sub_frames(fr, n_frames)
+    sx, sy, dia, ns = O.state_grid(*grid)
+    S = len(sx)
+    sel = np.linspace(0, S - 1, n_states).astype(int)
     rng = np.random.default_rng(0)
     model = rng.random(size * size) + 1j * rng.random(size * size)
     t = time.time()
-    O.estep_prob(model, size, (sx[sel], sy[sel], dia[sel]), num_rot, ofr)
-    el = time.time() - t
-    return dict(value=n_frames * num_rot * n_states / el, unit='frame*rotation*state evals/s',
-                cores=1, kind='port',
-                sample='E-step only (hologram, rotated log-views, sparse LL, argmax) of %d '
-                       'frames x %d states x %d rotations of the same workload, NumPy FP64 '
-                       'single thread, %.1f s; MaxLP excluded' %
-                       (n_frames, n_states, num_rot, el))
+    rmax, _, _, _, _ = CO.estep_top2(model, size, (sx[sel], sy[sel], dia[sel]), num_rot, ofr,
+                                     nthreads=nth)
+    t_e = time.time() - t
+    # MaxLP at the sample's answer
+    st = rmax // num_rot
+    vals = (sx[sel][st], sy[sel][st], dia[sel][st], (rmax % num_rot) * np.pi / num_rot)
+    mc = CO.MaxLPC(ofr, size)
+    mc.set_frames(*vals)
+    ring = np.nonzero(mc.rmask)[0]
+    mask = np.nonzero(mc.mask)[0]
+    pr = ring[np.linspace(0, len(ring) - 1, n_pixels).astype(int)]
+    pm = mask[np.linspace(0, len(mask) - 1, n_pixels).astype(int)]
+    t = time.time()
+    f1, resc, _, _ = mc.pixels(model[pr], pr, None, nth)
+    t_m1 = time.time() - t
+    t = time.time()
+    mc.pixels(model[pm], pm, float(np.mean(resc)), nth)
+    t_m3 = time.time() - t
+    est = t_e * (N / n_frames) * (S / n_states)
+    msp = (t_m1 * len(ring) + t_m3 * len(mask)) / n_pixels * (N / n_frames)
+    t_iter = est + msp
+    return dict(value=N * num_rot * S / t_iter, unit='frame*rotation*state evals/s', cores=nth,
+                kind='port', extrapolated=True,
+                seconds_per_iteration_extrapolated=t_iter,
+                estep_evals_per_s=n_frames * n_states * num_rot / t_e,
+                sample='oracle C restatement (FP64, OpenMP, %d threads): E-step of %d frames x '
+                       '%d states x %d rotations of the same workload (%.1f s) and MaxLP on %d '
+                       'ring + %d mask pixels of those frames (%.1f s); a full iteration '
+                       '(%d frames, %d states, %d ring + %d mask pixels) extrapolated '
+                       'linearly: E-step %.0f s + MaxLP %.0f s' %
+                       (nth, n_frames, n_states, num_rot, t_e, n_pixels, n_pixels,
+                        t_m1 + t_m3, N, S, len(ring), len(mask), est, msp),
+                host=host_facts())
+
+
+def cpu_config1(args):
+    """The reference's CPU configuration in full with the oracle's C restatement: 64^2,
+    2000 frames (150 photons/frame), 6069 states x 90 rotations, 3 EMC+MaxLP iterations
+    from a seeded random model."""
+    from oracle import c_oracle as CO
+    from oracle import emc_oracle as O
+    from refspi_amd import synth
+    CO.load()
+    nth = CO.threads()
+    cfg = synth.SynthConfig(**{**synth.CONFIG_CPU64.__dict__, 'seed': args.seed})
+    fr = synth.make_frames(cfg)
+    ofr = O.SparseFramesNP(fr.ones, fr.multi, fr.place_ones, fr.place_multi, fr.count_multi,
+                           fr.num_pix)
+    sx, sy, dia, ns = O.state_grid((-4., 4., 17), (-4., 4., 17), (7., 9., 21))
+    R = 90
+    rng = np.random.default_rng(args.seed)
+    model = rng.random(cfg.size ** 2) + 1j * rng.random(cfg.size ** 2)
+    mc = CO.MaxLPC(ofr, cfg.size)
+    its = []
+    for it in range(3):
+        t = time.time()
+        rmax, _, _, _, _ = CO.estep_top2(model, cfg.size, (sx, sy, dia), R, ofr, nthreads=nth)
+        t1 = time.time()
+        vals = O.unravel_rmax(rmax, ns, R, sx, sy, dia)
+        model, rescale, _ = mc.run_phaser(model, *vals, nthreads=nth)
+        t2 = time.time()
+        its.append(dict(estep_s=t1 - t, maxlp_s=t2 - t1, rescale=rescale))
+        log('cpu config 1 iteration %d: E-step %.1f s, MaxLP %.1f s' % (it + 1, t1 - t, t2 - t1))
+    tot = sum(i['estep_s'] + i['maxlp_s'] for i in its)
+    return dict(workload='64x64 q-grid, 2000 frames, 6069 states x 90 rotations, 3 EMC+MaxLP '
+                         'iterations (the reference\'s CPU configuration), oracle C '
+                         'restatement FP64, %d threads' % nth,
+                iterations=its, seconds_per_iteration=tot / 3,
+                value=cfg.num_data * R * len(sx) / (tot / 3),
+                unit='frame*rotation*state evals/s', host=host_facts())
+
+
+def frc_report(model, cfg):
+    """FRC (calc_frc.py:77-117) of a reconstruction against the generator's object
+    transform, with the best in-plane rotation; mean over rings 4 .. size//2."""
+    import contextlib
+    from refspi_amd import frc, synth
+    truth = (synth.model_scales(cfg)[0] * synth.object_ft(cfg)).reshape(cfg.size, cfg.size)
+    f = frc.FRC(np.asarray(model).reshape(cfg.size, cfg.size), truth)
+    with contextlib.redirect_stdout(sys.stderr):
+        r, c = f.calc_rot(num_rot=360, radmin=4)
+    cen = cfg.size // 2
+    return dict(mean_4_to_edge=float(c[4:cen].mean()), best_angle_deg=float(f.best_angle),
+                ring_frc=[round(float(v), 4) for v in c[:cen]],
+                note='FRC of the model after the run against the generator\'s object '
+                     'transform (calc_frc.py:77-117, best of 360 rotations)')
 
 
 def _frames_from_torch(d):
@@ -308,6 +418,14 @@ def stream_leg(args):
 
 
 def main():
+    # the JSON line is the only output on stdout: the library's progress prints go to stderr
+    import contextlib
+    with contextlib.redirect_stdout(sys.stderr):
+        out = run()
+    print(json.dumps(out), flush=True)
+
+
+def run():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=2)
@@ -318,16 +436,19 @@ def main():
     ap.add_argument('--banks', type=int, default=None,
                     help='LDS bank model of the photon scheduler (0 = none, 32, 64)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-frames', type=int, default=1000)
-    ap.add_argument('--cpu-states', type=int, default=30)
+    ap.add_argument('--cpu-frames', type=int, default=2000)
+    ap.add_argument('--cpu-states', type=int, default=256)
+    ap.add_argument('--cpu-pixels', type=int, default=64)
     ap.add_argument('--stream-frames', type=int, default=500_000)
     ap.add_argument('--stream-chunk', type=int, default=50_000)
     ap.add_argument('--refine-steps', type=int, default=3)
     ap.add_argument('--judge-frames', type=int, default=512)
-    ap.add_argument('--workload', choices=('all', 'joint', 'split', 'adjoint', 'stream'),
+    ap.add_argument('--workload', choices=('all', 'joint', 'split', 'adjoint', 'stream',
+                                           'cpu-config1'),
                     default='all',
                     help='joint: the EMC+MaxLP iteration (the headline value); split: the '
-                         'rotation-form split-BF16 vs FP32 E-step; adjoint: loss + gradient')
+                         'rotation-form split-BF16 vs FP32 E-step; adjoint: loss + gradient; '
+                         'cpu-config1: the 64^2 CPU configuration in full on the host cores')
     ap.add_argument('--split-frames', type=int, default=200_000)
     ap.add_argument('--split-rot', type=int, default=720)
     ap.add_argument('--split-reps', type=int, default=3)
@@ -342,19 +463,19 @@ def main():
         torch.cuda.synchronize()
         a.copy_(b)
         torch.cuda.synchronize()
-        print(json.dumps({'copy_bytes_read': 1 << 30, 'copy_bytes_written': 1 << 30}))
-        return
+        return {'copy_bytes_read': 1 << 30, 'copy_bytes_written': 1 << 30}
     if args.gpus != 1:
-        print('bench.py runs on one GPU only: this build has no multi-GPU path '
-              '(asked for --gpus %d)' % args.gpus, file=sys.stderr)
+        log('bench.py runs on one GPU only: this build has no multi-GPU path '
+            '(asked for --gpus %d)' % args.gpus)
         sys.exit(2)
+    if args.workload == 'cpu-config1':
+        return {'cpu_config1': cpu_config1(args)}
     import torch
     from refspi_amd.dset import Dataset
     from refspi_amd.emc import EMC
     if args.workload in ('split', 'adjoint', 'stream'):
         leg = dict(split=split_leg, adjoint=adjoint_leg, stream=stream_leg)[args.workload](args)
-        print(json.dumps({args.workload: leg}), flush=True)
-        return
+        return {args.workload: leg}
     grid = ((-4., 4., 17), (-4., 4., 17), (7., 9., 21))
     cfg, fr = make_dataset(args)
     size = cfg.size
@@ -375,6 +496,7 @@ def main():
     gather_ms = tables_ms = 0.
     batches = 0
     lds_rw = 0.
+    mx_products = mx_ms = 0.
     t0 = time.time()
     for i in range(args.steps):
         ts = time.time()
@@ -384,6 +506,9 @@ def main():
         tables_ms += tm['tables_ms']
         batches += tm['batches']
         lds_rw += H * (tm['lds_read_bytes_per_hyp'] + tm['lds_write_bytes_per_hyp'])
+        mi = emc.phaser.info()
+        mx_products += mi['entry_candidate_products']
+        mx_ms += mi['pattern_ms']
         log('step %d: %.2f s %s gather %.1f ms tables %.1f ms' %
             (i, time.time() - ts, emc.timings, tm['gather_ms'], tm['tables_ms']))
     torch.cuda.synchronize()
@@ -391,24 +516,26 @@ def main():
     t_iter = el / args.steps
     n = fr.ones.shape[0]
     value = n * args.num_rot * S / t_iter
-    # dominant kernel: the sparse log-likelihood gather.  Algorithmic bytes per frame x
-    # hypothesis evaluation = the frame's sparse record, 4*ones + 8*multi bytes.
-    rec_bytes = 4. * fr.ones.sum() + 8. * fr.multi.sum()
-    alg_bytes_per_step = H * rec_bytes
+    # dominant kernel: the sparse log-likelihood gather.  Its photons stay in VGPRs across
+    # hypotheses and the tables stream through LDS, so the binding resource is LDS: the
+    # algorithmic work per frame x hypothesis is one 4-byte LDS read per scored photon.
     gather_s = gather_ms / 1e3
-    achieved = alg_bytes_per_step * args.steps / gather_s / 1e9
+    avg_launch_s = gather_s / max(batches, 1)       # one launch = one batch's slot classes
     pm = emc.probmask
     scored = (pm[fr.place_ones] == 0).sum() + fr.count_multi[pm[fr.place_multi] == 0].sum()
-    lds_bytes = H * 4. * scored
-    lds_gbs = lds_bytes * args.steps / gather_s / 1e9
+    lds_bytes_per_step = H * 4. * scored
+    lds_gbs = lds_bytes_per_step * args.steps / gather_s / 1e9
+    lds_exec_gbs = lds_rw / gather_s / 1e9
+    # the sparse records calc_prob_all re-reads per hypothesis (4 ones + 8 multi bytes)
+    rec_bytes = 4. * fr.ones.sum() + 8. * fr.multi.sum()
+    rec_gbs = H * rec_bytes * args.steps / gather_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, 'profiles', 'gather_traffic_185.json')
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get('bytes_per_launch')
-    avg_launch_s = gather_s / max(batches, 1)
     hbm_gbs = traffic / avg_launch_s / 1e9 if traffic else None
-    lds_exec_gbs = lds_rw / gather_s / 1e9
+    mx_tflops = 6. * mx_products / (mx_ms / 1e3) / 1e12 if mx_ms > 0 else None
     out = dict(
         metric='frame x rotation evals/s per EMC+MaxLP iteration at 185^2 (x states, joint '
                'search)',
@@ -421,24 +548,36 @@ def main():
                     frames=n, states=S, rotations=args.num_rot, size=size,
                     estep_s=emc.timings.get('estep'), maxlp_s=emc.timings.get('maxlp'),
                     estep_stats=emc.estep_stats()),
-        roofline=dict(bound='hbm', achieved=achieved, peak=HBM_PEAK_GBS, unit='GB/s',
-                      frac=achieved / HBM_PEAK_GBS, traffic=traffic,
-                      kernel='k_gather', launches=batches,
+        roofline=dict(bound='lds', kernel='k_gather', unit='GB/s',
+                      achieved=lds_gbs, peak=LDS_PEAK_GBS, frac=lds_gbs / LDS_PEAK_GBS,
+                      traffic=traffic, launches=batches,
                       avg_launch_ms=gather_ms / max(batches, 1),
+                      what='one 4-byte LDS read per scored photon per frame x hypothesis '
+                           '(%.0f scored photons per step-hypothesis), / gather time' % scored,
+                      peak_what='256 CUs x 128 B/clk (32 banks x 4 B, the ds_read_b32 ceiling) '
+                                'x 2.4 GHz; measured conflict-free ds_read_b32: 27 of 32 '
+                                'lanes per clock',
+                      executed_gbs=lds_exec_gbs, executed_frac=lds_exec_gbs / LDS_PEAK_GBS,
+                      executed_what='every slot read (bank-schedule zero pads included) + the '
+                                    'table loads\' LDS writes',
                       hbm_measured_gbs=hbm_gbs,
                       hbm_measured_frac=hbm_gbs / HBM_PEAK_GBS if hbm_gbs else None,
-                      note='algorithmic bytes = H x sum_d(4 ones_d + 8 multi_d) per step (the '
-                           'sparse records calc_prob_all re-reads per hypothesis); here the '
-                           'photons stay in VGPRs across hypotheses, so achieved/peak is not '
-                           'an HBM utilisation: the HBM rate actually drawn is '
-                           'hbm_measured_gbs (traffic / avg launch), and the binding resource '
-                           'is LDS (see lds)'),
-        lds=dict(achieved=lds_gbs, peak=LDS_PEAK_GBS, unit='GB/s', frac=lds_gbs / LDS_PEAK_GBS,
-                 what='scored-photon gathers (4 B each) per step / gather time',
-                 executed_gbs=lds_exec_gbs, executed_frac=lds_exec_gbs / LDS_PEAK_GBS,
-                 executed_what='LDS bytes the gather executes: every slot read (bank-schedule '
-                               'zero pads included) + the table loads\' LDS writes'),
+                      hbm_what='traffic (rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, '
+                               'calibrated by a 1 GiB copy) / avg launch',
+                      record_equivalent_gbs=rec_gbs,
+                      record_what='sparse records (4 ones + 8 multi bytes) per frame x '
+                                  'hypothesis, as calc_prob_all re-reads them, / gather time: '
+                                  'a rate of useful work, not a bandwidth drawn'),
+        maxlp_roofline=dict(bound='valu_fp64', kernel='k_pattern', unit='TFLOP/s',
+                            achieved=mx_tflops, peak=78.6,
+                            frac=mx_tflops / 78.6 if mx_tflops else None,
+                            ms_per_step=mx_ms / args.steps,
+                            what='6 FP64 operations per entry x candidate product (c + R_d, '
+                                 '|.|^2, the running product) / pattern-search time, both '
+                                 'phases; %d products per step' % (mx_products / args.steps)),
     )
+    out['frc'] = frc_report(emc.model, cfg)
+    out['frc']['iterations'] = args.warmup + args.steps
     if args.workload == 'all':
         del emc
         torch.cuda.empty_cache()
@@ -447,8 +586,8 @@ def main():
         out['streamed_two_class'] = stream_leg(args)
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(fr, size, args.num_rot, grid, args.cpu_frames,
-                                           args.cpu_states)
-    print(json.dumps(out), flush=True)
+                                           args.cpu_states, args.cpu_pixels)
+    return out
 
 
 if __name__ == '__main__':
