@@ -283,8 +283,12 @@ def adjoint_leg(args):
     plan = RotEM(Dataset(frames=fr), cfg.size, R, rel_scale=rel, mode='fp64', full_turn=True)
     out = dict(workload='loss + gradient (model, shifts, diameters), %dx%d q-grid, %d frames, '
                         '%d rotations' % (cfg.size, cfg.size, cfg.num_data, R))
-    for mode in ('fp64', 'split'):
-        plan.set_mode(mode)
+    for mode in ('fp64', 'split', 'split_gemms'):
+        # split_gemms: split-BF16 forward and the adjoint's two dense products (Obar = P A,
+        # GT = Q^T B) in split BF16 on MFMA as well
+        plan.set_mode('split' if mode == 'split_gemms' else mode)
+        plan.set_frame_grad('split' if mode == 'split_gemms' else 'fp64')
+        plan.set_model_grad('split' if mode == 'split_gemms' else 'fp64')
         plan.loss_grad(model, sx, sy, dia, sc)       # warm-up (allocations)
         torch.cuda.synchronize()
         ts = []

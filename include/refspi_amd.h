@@ -239,7 +239,10 @@ int rsp_rot_set_frames(rsp_rot* plan, int64_t num_data, const int32_t* ones,
  * 4096 for the adjoint), 3 = number of object classes C (default 1), 5 = per-kernel
  * timing (read with rsp_rot_timing), 6 = precision of the frame-gradient GEMM
  * Obar = P O of the adjoint (0 FP64 rocBLAS = default, 1 split BF16 MFMA: about 1e-5
- * relative in Obar, for gradient refinement).  Refused while a pass is open. */
+ * relative in Obar, for gradient refinement), 7 = precision of the model-gradient GEMM
+ * GT = Q^T B of the adjoint (Q = prob * scale, B = the reference rows; 0 FP64 rocBLAS =
+ * default, 1 split BF16 MFMA with FP32 accumulation over each sub-chunk of at most 12288
+ * frames, sub-chunks added in FP64).  Refused while a pass is open. */
 int rsp_rot_set_option(rsp_rot* plan, int key, int64_t value);
 /* model_c: device c16 npix; sx, sy, dia, scales: device f64 N; ll, prob: device f64 N*R
  * (either may be NULL); logz: device f64 N (may be NULL).  Synchronises `stream`. */

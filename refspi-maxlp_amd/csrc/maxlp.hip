@@ -42,6 +42,7 @@ constexpr int kThreads = 256;
 constexpr int kPerThread = 5;
 constexpr int kWaves = kThreads / 64;
 constexpr int kChunk = 512;       // entries staged in LDS at a time
+constexpr int kMaxPiece = 8;      // largest photon count of one model-space entry
 
 struct Entry {
   double rr, ri, k;
@@ -139,9 +140,15 @@ __global__ __launch_bounds__(256) void k_pull(Geo g, const int32_t* __restrict__
         const int t = uu * g.size + vv;
         if (!mask[t]) continue;
         if (detector_index(g, t, c, s) != p) continue;
-        if (mode)
-          keys[w++] = ((uint64_t)t << 47) | ((uint64_t)d << 22) | (uint64_t)cnt[e];
-        ++n;
+        // a count above 8 becomes pieces of at most 8 (k_pattern raises each factor to its
+        // piece's count between renormalisations; the pieces multiply to the same power)
+        int left = cnt[e];
+        while (left > 0) {
+          const int piece = left < kMaxPiece ? left : kMaxPiece;
+          if (mode) keys[w++] = ((uint64_t)t << 47) | ((uint64_t)d << 22) | (uint64_t)piece;
+          ++n;
+          left -= piece;
+        }
       }
   if (!mode) nmatch[e] = n;
 }
@@ -242,26 +249,6 @@ __device__ __forceinline__ bool cand_better(double a, int ia, double b, int ib) 
   return a > b || (a == b && ia < ib);
 }
 
-// prod * a^K as m * 2^xs without leaving FP64's normal range for any K and a: a = m_a 2^e
-// contributes K*e exactly and m_a^K (m_a in [0.5, 1)) is formed with a renormalisation
-// every 8 factors.
-__device__ __forceinline__ double pow_renorm(double prod, double a, int K, int* xs) {
-  int ea;
-  const double ma = frexp(a, &ea);
-  double pr = prod;
-  int x = ea * K;
-  for (int k = 0; k < K; ++k) {
-    pr = pr * ma;
-    if ((k & 7) == 7) {
-      int e;
-      pr = frexp(pr, &e);
-      x += e;
-    }
-  }
-  *xs = x;
-  return pr;
-}
-
 // mean_d |c + R_d|^2 from the moments (no cancellation: see k_moments)
 __device__ __forceinline__ double mean_w(double cr, double ci, double m1r, double m1i, double v) {
   const double ar = cr + m1r, ai = ci + m1i;
@@ -275,7 +262,7 @@ __device__ __forceinline__ double mean_w(double cr, double ci, double m1r, doubl
 // kProbe: one evaluation of the caller's candidates probe_cand[0..probe_n) at pixel probe_t
 // (get_logq_pixel, maxLP.py:181-197) written to probe_out; same arithmetic as the search.
 template <bool kProbe>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_pattern(
+__global__ __launch_bounds__(kThreads) void k_pattern(
     const uint64_t* __restrict__ order, const int32_t* __restrict__ pixels,
     const int64_t* __restrict__ seg_beg,
     const int64_t* __restrict__ seg_end, const Entry* __restrict__ ent,
@@ -354,37 +341,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 4))
             const double re = cr[u] + E.rr, im = ci[u] + E.ri;
             prod[u] = prod[u] * (re * re + im * im);
           }
-          // renormalise every 8 factors (scaling by powers of two is exact, so the period
-          // does not change the product's rounding while 8 factors stay within FP64's
-          // normal range: |f_c + R_d|^2 in [2^-120, 2^120]; an exact zero stays zero)
-          if ((i & 7) == 7) {
-#pragma unroll
-            for (int u = 0; u < kPerThread; ++u) {
-              int e;
-              prod[u] = frexp(prod[u], &e);
-              ex[u] += e;
-            }
-          }
         } else {
-          // a^K, then one renormalisation.  With K <= 8 and a in [2^-120, 2^120] the K
-          // products stay within FP64's normal range (prod is in [0.5, 1) here); otherwise
-          // a = m 2^e (m in [0.5, 1)) contributes K*e to the exponent exactly and m^K is
-          // formed with a renormalisation every 8 factors.  Scaling by powers of two is
-          // exact, so both forms give the same bits wherever the first stays normal.
+          // 2 <= K <= 8 (k_pull splits larger counts): a^K directly, renormalised below
 #pragma unroll
           for (int u = 0; u < kPerThread; ++u) {
             const double re = cr[u] + E.rr, im = ci[u] + E.ri;
             const double a = re * re + im * im;
             double pr = prod[u] * a;
-            int xs = 0;
-            if (K <= 8 && a >= 0x1p-120 && a <= 0x1p120) {
-              for (int k = 1; k < K; ++k) pr = pr * a;
-            } else {
-              pr = pow_renorm(prod[u], a, K, &xs);
-            }
+            for (int k = 1; k < K; ++k) pr = pr * a;
+            prod[u] = pr;
+          }
+        }
+        // renormalise every 8 factors and after every K > 1 entry: at most 8 factors between
+        // renormalisations, so the products stay within FP64's normal range whenever
+        // |f_c + R_d|^2 is in [2^-120, 2^120] (scaling by powers of two is exact, so the
+        // period does not change the product's rounding; an exact zero stays zero, giving
+        // -inf as log 0 does in the reference)
+        if ((i & 7) == 7 || K > 1) {
+#pragma unroll
+          for (int u = 0; u < kPerThread; ++u) {
             int e;
-            prod[u] = frexp(pr, &e);
-            ex[u] += xs + e;
+            prod[u] = frexp(prod[u], &e);
+            ex[u] += e;
           }
         }
       }

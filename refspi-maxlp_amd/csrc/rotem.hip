@@ -570,6 +570,58 @@ __global__ void k_ot_planes(const double2* __restrict__ OT, int K, int H, int Hs
   lo[(2 * j + 1) * Hp + h] = (__bf16)(im - (float)ih);
 }
 
+// A planes of the split-BF16 model-gradient GEMM GT = Q^T B: row h holds Q[d, h] over the
+// sub-chunk's frames d = 0 .. Kc-1 (zero for h >= H or d >= n), rounded to FP32 and split
+// into bf16 hi + lo.  Every element of the Mp x Kc planes is written.
+__global__ void k_qt_planes(const double* __restrict__ Q, int64_t n, int H, int64_t Mp,
+                            int64_t Kc, __bf16* __restrict__ hi, __bf16* __restrict__ lo) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= Mp * Kc) return;
+  const int64_t h = t / Kc, d = t - h * Kc;
+  const float v = (h < H && d < n) ? (float)Q[d * H + h] : 0.f;
+  const __bf16 vh = (__bf16)v;
+  hi[t] = vh;
+  lo[t] = (__bf16)(v - (float)vh);
+}
+
+// B planes of GT = Q^T B: the transpose of the sub-chunk's FP64 reference rows B [n x ldb]
+// (columns >= kval are zero) into rows k = 0 .. Np-1 over frames d = 0 .. Kc-1, through a
+// 64 x 64 LDS tile (reads along k and writes along d both coalesced); every element of the
+// Np x Kc planes is written (zero outside k < kval, d < n).
+__global__ __launch_bounds__(256) void k_bt_planes(const double* __restrict__ B, int64_t n,
+                                                   int64_t ldb, int64_t kval, int64_t Np,
+                                                   int64_t Kc, __bf16* __restrict__ hi,
+                                                   __bf16* __restrict__ lo) {
+  __shared__ double tile[64][65];
+  const int64_t k0 = (int64_t)blockIdx.x * 64, d0 = (int64_t)blockIdx.y * 64;
+  const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+  for (int i = 0; i < 16; ++i) {
+    const int r = r0 + 4 * i;
+    const int64_t d = d0 + r, k = k0 + c;
+    tile[r][c] = (d < n && k < kval) ? B[d * ldb + k] : 0.;
+  }
+  __syncthreads();
+  for (int i = 0; i < 16; ++i) {
+    const int r = r0 + 4 * i;
+    const int64_t k = k0 + r, d = d0 + c;
+    if (k >= Np || d >= Kc) continue;
+    const float v = (float)tile[c][r];
+    const __bf16 vh = (__bf16)v;
+    hi[k * Kc + d] = vh;
+    lo[k * Kc + d] = (__bf16)(v - (float)vh);
+  }
+}
+
+// GT[k * H + h] = (accumulate ? GT : 0) + C[k * ldc + h] in FP64, k < Kp, h < H.
+__global__ void k_gt_accum(const float* __restrict__ C, int64_t ldc, int64_t Kp, int H,
+                           int accumulate, double* __restrict__ GT) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= Kp * H) return;
+  const int64_t k = t / H, h = t - k * H;
+  const double v = (double)C[k * ldc + h];
+  GT[t] = accumulate ? GT[t] + v : v;
+}
+
 // One work-group per frame of [d0, d0 + gridDim.x): frame gradients (sx, sy, dia).
 // Dense part: threads over the scored pixels (Obar row dl of the sub-chunk).  Sparse part:
 // wave w takes photons w, w+4, ...; lanes over hypotheses h = lane + 64 t.
@@ -837,6 +889,13 @@ struct rsp_rot {
   int fg_split = 0;
   int64_t fg_Mp = 0, fg_Np = 0, fg_Hp = 0;
   __bf16 *d_fgA = nullptr, *d_fgB = nullptr;
+  // model gradient with a split-BF16 GT = Q^T B (option 7): A planes [2 x gt_Mp x gt_Kc]
+  // of Q^T (r = hypothesis, k = frame of the sub-chunk), B planes [2 x gt_Np x gt_Kc] of
+  // B^T (r = pixel component), FP32 product [Kp x gt_Mp]
+  int gt_split = 0;
+  int64_t gt_Mp = 0, gt_Np = 0, gt_Kc = 0;
+  __bf16 *d_gtA = nullptr, *d_gtB = nullptr;
+  float* d_gtC = nullptr;
   // streamed pass
   int pass_flags = -1;
   int64_t pass_max = 0, pc_alloc = 0;
@@ -878,6 +937,9 @@ void free_adj(rsp_rot* p) {
   dfree(p->d_aGO);
   dfree(p->d_fgA);
   dfree(p->d_fgB);
+  dfree(p->d_gtA);
+  dfree(p->d_gtB);
+  dfree(p->d_gtC);
   p->adj_chunk = 0;
 }
 
@@ -946,6 +1008,21 @@ int ensure_adj(rsp_rot* p, int64_t chunk) {
     return rc;
   }
   RSP_HIP(hipMemset(p->d_fgA, 0, sizeof(__bf16) * (size_t)(2 * p->fg_Mp * p->fg_Hp)));
+  if (p->gt_split) {
+    p->gt_Mp = xterm_pad_m(p->H);
+    p->gt_Np = xterm_pad_n(p->Kp);
+    p->gt_Kc = xterm_pad_k(chunk);
+    if (xterm_split_k(p->gt_Kc) != 1) {
+      free_adj(p);
+      return fail(kErrSize, "split model-gradient GEMM: adjoint chunk above 12288 frames");
+    }
+    if ((rc = dalloc(&p->d_gtA, 2 * p->gt_Mp * p->gt_Kc)) ||
+        (rc = dalloc(&p->d_gtB, 2 * p->gt_Np * p->gt_Kc)) ||
+        (rc = dalloc(&p->d_gtC, p->Kp * p->gt_Mp))) {
+      free_adj(p);
+      return rc;
+    }
+  }
   // operand padding (rows >= H of A, columns >= 2K of A and B) stays zero
   RSP_HIP(hipMemset(p->d_aA, 0, sizeof(double) * (size_t)(p->Mpad * p->Kp)));
   RSP_HIP(hipMemset(p->d_aB, 0, sizeof(double) * (size_t)(chunk * p->Kp)));
@@ -1187,9 +1264,29 @@ int run_adj_frames(rsp_rot* p, const Csr& f, const double* prob, const double* s
                          st, prob, scales, d0, n, p->H, p->d_aQ);
       RSP_LAUNCH_CHECK("k_scaled_prob");
       // GT (H x Kp, column-major) (+)= Q^T B : GT[k * H + h] = sum_d Q[d, h] B[d, k]
-      if ((rc = xterm_dgemm(p->blas, false, true, p->H, p->Kp, n, 1., p->d_aQ, p->H, p->d_aB, p->Kp,
-                            (gt_first && d0 == 0) ? 0. : 1., p->d_aGT, p->H, st)))
+      const bool first = gt_first && d0 == 0;
+      if (p->gt_split) {
+        // on MFMA in split BF16: r = h, d = k (pixel component), reduction over frames
+        const int64_t nq = p->gt_Mp * p->gt_Kc;
+        hipLaunchKernelGGL(k_qt_planes, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, st,
+                           p->d_aQ, n, p->H, p->gt_Mp, p->gt_Kc, p->d_gtA, p->d_gtA + nq);
+        RSP_LAUNCH_CHECK("k_qt_planes");
+        const int64_t nb = p->gt_Np * p->gt_Kc;
+        hipLaunchKernelGGL(k_bt_planes, dim3((unsigned)(p->gt_Np / 64), (unsigned)ceil_div(p->gt_Kc, 64)),
+                           dim3(256), 0, st, p->d_aB, n, p->Kp, 2 * (int64_t)p->K, p->gt_Np,
+                           p->gt_Kc, p->d_gtB, p->d_gtB + nb);
+        RSP_LAUNCH_CHECK("k_bt_planes");
+        if ((rc = xterm_gemm_split(p->d_gtA, p->d_gtA + nq, p->d_gtB, p->d_gtB + nb, p->d_gtC,
+                                   p->gt_Mp, p->Kp, p->gt_Kc, p->gt_Mp, 0, st)))
+          return rc;
+        const int64_t ng = p->Kp * p->H;
+        hipLaunchKernelGGL(k_gt_accum, dim3((unsigned)ceil_div(ng, 256)), dim3(256), 0, st,
+                           p->d_gtC, p->gt_Mp, p->Kp, p->H, first ? 0 : 1, p->d_aGT);
+        RSP_LAUNCH_CHECK("k_gt_accum");
+      } else if ((rc = xterm_dgemm(p->blas, false, true, p->H, p->Kp, n, 1., p->d_aQ, p->H,
+                                   p->d_aB, p->Kp, first ? 0. : 1., p->d_aGT, p->H, st))) {
         return rc;
+      }
     }
     if (want_frame) {
       if (p->fg_split) {
@@ -1488,6 +1585,14 @@ int rsp_rot_set_option(rsp_rot* p, int key, int64_t value) {
     case 6:
       if (value < 0 || value > 1) return fail(kErrSize, "frame-gradient GEMM must be 0 or 1");
       p->fg_split = (int)value;
+      return kOk;
+    case 7:
+      if (value < 0 || value > 1) return fail(kErrSize, "model-gradient GEMM must be 0 or 1");
+      if (value != p->gt_split) {
+        RSP_HIP(hipSetDevice(p->device));
+        free_adj(p);            // its buffers are (re)allocated on the next adjoint
+      }
+      p->gt_split = (int)value;
       return kOk;
     default:
       return fail(kErrUnsupported, "unknown option key");

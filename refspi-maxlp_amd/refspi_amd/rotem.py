@@ -83,6 +83,15 @@ class _Plan:
         _lib.call('rsp_rot_set_option', self._h, 6, 1 if precision == 'split' else 0)
         self.frame_grad = precision
 
+    def set_model_grad(self, precision):
+        """Precision of the model-gradient GEMM GT = Q^T B of the adjoint (Q = P * scale,
+        B = the reference rows): 'fp64' (rocBLAS, the default) or 'split' (BF16 hi/lo on
+        MFMA with FP32 accumulation over each sub-chunk's frames, added in FP64)."""
+        if precision not in ('fp64', 'split'):
+            raise ValueError("model-gradient precision must be 'fp64' or 'split'")
+        _lib.call('rsp_rot_set_option', self._h, 7, 1 if precision == 'split' else 0)
+        self.model_grad = precision
+
     def __del__(self):
         h = getattr(self, '_h', None)
         if h is not None and _lib._lib is not None:

@@ -239,3 +239,38 @@ def test_split_frame_gradients_close_to_fp64():
         rel_err = np.linalg.norm(a - b) / np.linalg.norm(b)
         print('frame gradient relative error %.3g' % rel_err, file=sys.stderr)
         assert rel_err < 1e-4   # measured 1.4e-6
+
+
+def test_split_model_gradient_gemm_close_to_fp64():
+    """Option 7: the model-gradient GEMM GT = Q^T B in split BF16 on MFMA (each sub-chunk's
+    frames accumulated in FP32, sub-chunks added in FP64) gives the FP64 adjoint's model
+    gradient to a small norm-wise relative error (185^2, 5000 frames over two sub-chunks,
+    360 rotations); the frame gradients do not use GT and are unchanged bit for bit."""
+    import torch
+    cfg, fr = _cfg4(5000)
+    size, R = 185, 360
+    t = fr.truth
+    rng = np.random.default_rng(22)
+    model = 0.3 * (rng.standard_normal(size * size) + 1j * rng.standard_normal(size * size))
+    N = fr.num_data
+    sx = t['shifts'][:, 0] + rng.normal(0, 0.3, N)
+    sy = t['shifts'][:, 1] + rng.normal(0, 0.3, N)
+    dia = t['diameters'] + rng.normal(0, 0.2, N)
+    plan = _plan(fr, size, R, rel=1.)
+    dev = 'cuda'
+    dm = torch.as_tensor(model, device=dev)
+    dsx, dsy, ddia = (torch.as_tensor(v, device=dev) for v in (sx, sy, dia))
+    _, prob, _ = plan.estep_device(dm, dsx, dsy, ddia, want_ll=False)
+    ref = [v.cpu().numpy() for v in plan.adjoint_device(dm, dsx, dsy, ddia, None, prob)]
+    plan.set_model_grad('split')
+    got = [v.cpu().numpy() for v in plan.adjoint_device(dm, dsx, dsy, ddia, None, prob)]
+    for a, b in zip(got[1:], ref[1:]):
+        np.testing.assert_array_equal(a, b)
+    rel_err = np.linalg.norm(got[0] - ref[0]) / np.linalg.norm(ref[0])
+    worst = np.abs(got[0] - ref[0]).max() / np.abs(ref[0]).max()
+    print('model gradient relative error %.3g (largest entry error / largest entry %.3g)' %
+          (rel_err, worst), file=sys.stderr)
+    assert rel_err < 1e-4 and worst < 1e-4
+    plan.set_model_grad('fp64')
+    back = plan.adjoint_device(dm, dsx, dsy, ddia, None, prob)[0].cpu().numpy()
+    np.testing.assert_array_equal(back, ref[0])

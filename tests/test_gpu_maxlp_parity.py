@@ -150,9 +150,11 @@ def test_logq_pixel_at_100k_frames_within_bound():
 
 @pytest.mark.parametrize('count', [5, 100, 4000])
 def test_logq_pixel_high_counts_near_cancellation(count):
-    """Entries with K > 8 and factors |c + R_d|^2 far below 1: the product form takes K*e
-    of a = m 2^e into the exponent exactly (a^K alone would leave FP64's range, e.g.
-    (1e-4)^100), and agrees with the oracle's direct sum of K log w within the bound."""
+    """Entries with K > 8 and factors |c + R_d|^2 far below 1: the pull-back splits a count
+    into pieces of at most 8, each raised between renormalisations (a^K in one go would
+    leave FP64's range, e.g. (1e-4)^100), and the product form agrees with the oracle's
+    direct sum of K log w within the bound; get_photons_pixel still reports the whole
+    count."""
     from oracle import c_oracle as CO
     from oracle import emc_oracle as O
     from refspi_amd.dset import Dataset
