@@ -1,7 +1,8 @@
 """The HIP path against the committed golden fixture (tests/golden/golden_small.npz),
 without running the oracle: 21x21 q-grid, 24 frames, 27 states x 8 rotations, two
 EMC+MaxLP iterations (each started from the fixture's own input model), the kernels.cu
-mirrors, and the rotation form (E-step in three precisions, merge, adjoint).
+mirrors, the MaxLP photon pull table, and the rotation form (E-step in three precisions,
+merge, adjoint).
 
 Bars: argmax bit-exact; maxll, msums, vscale 1e-12 relative; the mirrors 1e-13; MaxLP
 1e-9 relative except at pixels whose FP64 pattern-search argmax was a near-tie (margin
@@ -86,6 +87,14 @@ def test_golden_iteration(g, it):
     rel = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-300)
     bad = np.nonzero(rel > 1e-9)[0]
     assert np.all(g[p + 'maxlp_margin'][bad] < 1e-12), bad
+    if it == 1:
+        # the fixture's photon pull table (maxLP.py:250-252 for iteration 1's angles): the
+        # GPU pull-back gives every (mask pixel, frame) the count at exactly that pixel
+        dense = Dataset(frames=_Frames(g))
+        K = np.stack([dense.get_frame(d) for d in range(dense.num_data)])
+        rows = np.arange(dense.num_data)
+        for t, idx in zip(g['pull_pixels'], g['pull_index']):
+            np.testing.assert_array_equal(emc.phaser.get_photons_pixel(int(t)), K[rows, idx])
 
 
 @pytest.fixture(scope='module')
