@@ -221,14 +221,18 @@ __global__ __launch_bounds__(256) void k_ref_rows(
       }
     }
     const int64_t k = dl * Kp + 2 * j0;
+    // streaming stores: the rows are read once, by the GEMM, after the whole chunk is
+    // written (far more than the caches hold)
     if constexpr (MODE == kFP32) {
       float4* o = reinterpret_cast<float4*>((float*)B + k);
-      o[0] = make_float4((float)vr[0], (float)vi[0], (float)vr[1], (float)vi[1]);
-      o[1] = make_float4((float)vr[2], (float)vi[2], (float)vr[3], (float)vi[3]);
+      __builtin_nontemporal_store(
+          make_float4((float)vr[0], (float)vi[0], (float)vr[1], (float)vi[1]), o);
+      __builtin_nontemporal_store(
+          make_float4((float)vr[2], (float)vi[2], (float)vr[3], (float)vi[3]), o + 1);
     } else if constexpr (MODE == kFP64) {
       double2* o = reinterpret_cast<double2*>((double*)B + k);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) o[u] = make_double2(vr[u], vi[u]);
+      for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(make_double2(vr[u], vi[u]), o + u);
     } else {
       __bf16* bh = (__bf16*)B;
       __bf16* bl = bh + Bpad * Kp;
@@ -242,8 +246,8 @@ __global__ __launch_bounds__(256) void k_ref_rows(
         lv[2 * u] = (__bf16)(fr - (float)hr);
         lv[2 * u + 1] = (__bf16)(fi - (float)hi);
       }
-      *reinterpret_cast<bf16x8*>(bh + k) = hv;
-      *reinterpret_cast<bf16x8*>(bl + k) = lv;
+      __builtin_nontemporal_store(hv, reinterpret_cast<bf16x8*>(bh + k));
+      __builtin_nontemporal_store(lv, reinterpret_cast<bf16x8*>(bl + k));
     }
   }
   red[threadIdx.x] = acc;
