@@ -54,6 +54,8 @@ namespace {
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef double f64x2v __attribute__((ext_vector_type(2)));
 constexpr double kPi = 3.14159265358979323846;
 
 enum Mode { kFP32 = 0, kSplit = 1, kFP64 = 2 };
@@ -224,15 +226,18 @@ __global__ __launch_bounds__(256) void k_ref_rows(
     // streaming stores: the rows are read once, by the GEMM, after the whole chunk is
     // written (far more than the caches hold)
     if constexpr (MODE == kFP32) {
-      float4* o = reinterpret_cast<float4*>((float*)B + k);
-      __builtin_nontemporal_store(
-          make_float4((float)vr[0], (float)vi[0], (float)vr[1], (float)vi[1]), o);
-      __builtin_nontemporal_store(
-          make_float4((float)vr[2], (float)vi[2], (float)vr[3], (float)vi[3]), o + 1);
+      f32x4v* o = reinterpret_cast<f32x4v*>((float*)B + k);
+      const f32x4v a = {(float)vr[0], (float)vi[0], (float)vr[1], (float)vi[1]};
+      const f32x4v b = {(float)vr[2], (float)vi[2], (float)vr[3], (float)vi[3]};
+      __builtin_nontemporal_store(a, o);
+      __builtin_nontemporal_store(b, o + 1);
     } else if constexpr (MODE == kFP64) {
-      double2* o = reinterpret_cast<double2*>((double*)B + k);
+      f64x2v* o = reinterpret_cast<f64x2v*>((double*)B + k);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(make_double2(vr[u], vi[u]), o + u);
+      for (int u = 0; u < 4; ++u) {
+        const f64x2v v = {vr[u], vi[u]};
+        __builtin_nontemporal_store(v, o + u);
+      }
     } else {
       __bf16* bh = (__bf16*)B;
       __bf16* bl = bh + Bpad * Kp;
