@@ -113,7 +113,10 @@ int rsp_emc_set_frames(rsp_emc* plan, int64_t num_data, const int32_t* ones,
  *   7 = per-workgroup LDS layout of the table parts, fitted to the workgroup's frames
  *       (1 = on, the default; 0 = identity; 32-bank model and <= 2 parts; set before
  *       frames).  Only the slot layout and the table loads' addresses change, never the
- *       result. */
+ *       result.
+ *   8 = broadcast-aware photon slots (1 = on, the default; 0 = off; set before frames):
+ *       lanes that read the same table entry may share a bank in one slot (one LDS access
+ *       serves them).  Only the slot layout changes, never the result. */
 int rsp_emc_set_option(rsp_emc* plan, int key, int64_t value);
 
 /* Run the E-step for `model_c` (device, npix complex) with per-frame `scales` (device,
@@ -141,8 +144,13 @@ int rsp_emc_destroy(rsp_emc* plan);
 /* Host-side check of the photon slot scheduler and the per-workgroup table layouts (no
  * device calls; no counterpart in the reference: the reference has no slot schedule).
  * `trials` random lane x bank groups and trials/500 (at least 1) rounds of random tiles,
- * from `seed`.  out (host int64[5]): [0] groups, [1] slot conflicts, [2] groups whose slot
- * count exceeds the maximum degree, [3] layouts, [4] layout faults; [1], [2], [4] are 0
+ * from `seed`.  Each group is scheduled exactly (Koenig) and broadcast-aware (option 8).
+ * out (host int64[7]): [0] groups, [1] slot conflicts (a lane twice in a slot; a bank
+ * twice in a slot -- broadcast: two different entries of a bank in a slot), [2] groups
+ * whose exact slot count differs from the maximum degree or whose broadcast schedule is
+ * longer than the exact one, [3] layouts, [4] layout faults, [5] slots saved by the
+ * broadcast schedules, [6] broadcast slot count of a fixed half-wave whose 32 lanes share
+ * five entries of one bank plus one own entry (optimum 6, exact 161); [1], [2], [4] are 0
  * for a correct scheduler. */
 int rsp_emc_schedule_check(int64_t trials, uint64_t seed, int64_t* out);
 

@@ -23,6 +23,7 @@
 //                      log-likelihood of the chosen hypothesis, summed in the reference
 //                      order (ones then multi, kernels.cu:126-136).
 #include <algorithm>
+#include <bitset>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -797,6 +798,7 @@ struct rsp_emc {
   int64_t opt_batch = 0, opt_nseg = 0;
   int regroup_iters = 4000;  // local-search swaps per window of 8 lane groups (0: off)
   int layout = 1;            // per-tile LDS layout of the table parts (0: identity)
+  int bcast = 1;             // broadcast-aware slot schedule (0: distinct banks per slot)
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   struct Class {
     int cfg, n_tiles, gw0;
@@ -1021,6 +1023,239 @@ int schedule_group(const std::vector<std::vector<int>>& entries, int nbanks,
       bank_at[(size_t)b * D + k] = (l << 16) | r;
     }
   return D;
+}
+
+// Broadcast-aware slot schedule.  Lanes of a half-wave that read the same LDS entry in the
+// same slot take one bank access between them (gfx950 serves such a ds_read_b32 at the
+// conflict-free rate; two entries of one bank halve it), so a (bank, slot) may hold one
+// entry for any number of lanes.  The bound becomes max(longest lane, per bank the sum over
+// its entries of the most times one lane holds that entry).  Occurrence r of entry v in a
+// lane belongs to item (v, r), whose lanes should share a slot.  Items of two or more
+// lanes are placed first, largest first, each in the lowest slot free at its bank and all
+// its lanes (else in the free bank slot that takes the most of its lanes, the rest
+// following as a new piece); single-lane items, busiest banks first, then take the lowest
+// slot free at both their lane and bank, or the alternating-path exchange of
+// schedule_group restricted to single-lane items (a path that meets a shared slot is
+// abandoned and the next pair of slots tried).  Tries D = bound, bound + 1, .. < limit
+// slots; returns the slots used, or -1 (then slot_of is meaningless).
+using SlotSet = std::bitset<256>;
+
+int schedule_bcast_try(const std::vector<std::vector<int>>& entries, int nbanks, int D,
+                       const std::vector<std::pair<int, int>>& vl,
+                       std::vector<std::vector<int>>& slot_of) {
+  const int G = (int)entries.size();
+  SlotSet mask;
+  for (int k = 0; k < D; ++k) mask.set(k);
+  std::vector<SlotSet> lu(G), bu(nbanks);
+  // lane_at: -1 free, -2 shared item, else the bank of the single-lane item there;
+  // bank_at: -1 free, -2 shared item, else the lane of the single-lane item there
+  std::vector<int> lane_at((size_t)G * D, -1), bank_at((size_t)nbanks * D, -1);
+  std::vector<int64_t> sval((size_t)G * D, -1);  // item key of the single at (lane, slot)
+  std::vector<int> bload(nbanks, 0);
+  struct Item {
+    int v, n0, n;  // entry, its lanes ilane[n0 .. n0+n)
+    int r;         // occurrence number within each of those lanes
+  };
+  std::vector<Item> multi, single;
+  std::vector<int> ilane;
+  // vl is sorted by (entry, lane): a run of equal pairs is one lane's repeats of the entry
+  for (size_t i = 0; i < vl.size();) {
+    size_t j = i;
+    int rmax = 0;
+    while (j < vl.size() && vl[j].first == vl[i].first) {
+      size_t b = j;
+      while (b < vl.size() && vl[b] == vl[j]) ++b;
+      rmax = std::max(rmax, (int)(b - j));
+      j = b;
+    }
+    bload[vl[i].first % nbanks] += rmax;
+    for (int r = 0; r < rmax; ++r) {
+      Item it{vl[i].first, (int)ilane.size(), 0, r};
+      for (size_t a = i; a < j;) {
+        size_t b = a;
+        while (b < j && vl[b] == vl[a]) ++b;
+        if ((int)(b - a) > r) {
+          ilane.push_back(vl[a].second);
+          ++it.n;
+        }
+        a = b;
+      }
+      (it.n > 1 ? multi : single).push_back(it);
+    }
+    i = j;
+  }
+  std::stable_sort(multi.begin(), multi.end(), [&](const Item& a, const Item& b) {
+    return a.n != b.n ? a.n > b.n : bload[a.v % nbanks] > bload[b.v % nbanks];
+  });
+  std::stable_sort(single.begin(), single.end(), [&](const Item& a, const Item& b) {
+    return bload[a.v % nbanks] > bload[b.v % nbanks];
+  });
+  auto key = [](int v, int r) { return ((int64_t)v << 32) | (uint32_t)r; };
+  std::vector<std::vector<std::pair<int64_t, int>>> got(G);  // per lane: (item key, slot)
+  std::vector<int> L, rest;
+  for (const Item& it : multi) {
+    const int b = it.v % nbanks;
+    L.assign(ilane.begin() + it.n0, ilane.begin() + it.n0 + it.n);
+    while (!L.empty()) {
+      SlotSet c = ~bu[b] & mask;
+      for (int l : L) c &= ~lu[l];
+      int k = -1;
+      if (c.any()) {
+        k = (int)c._Find_first();
+      } else {
+        int best = 0;
+        for (int s = 0; s < D; ++s)
+          if (!bu[b][s]) {
+            int n = 0;
+            for (int l : L) n += !lu[l][s];
+            if (n > best) {
+              best = n;
+              k = s;
+            }
+          }
+        if (k < 0) return -1;
+      }
+      rest.clear();
+      for (int l : L) {
+        if (lu[l][k]) {
+          rest.push_back(l);
+          continue;
+        }
+        lu[l].set(k);
+        lane_at[(size_t)l * D + k] = -2;
+        got[l].push_back({key(it.v, it.r), k});
+      }
+      bu[b].set(k);
+      bank_at[(size_t)b * D + k] = -2;
+      L.swap(rest);
+    }
+  }
+  std::vector<std::pair<int, int>> path;  // (lane, bank) of each edge, colours a, c, a, ..
+  std::vector<int64_t> pv;
+  for (const Item& it : single) {
+    const int l = ilane[it.n0], b = it.v % nbanks;
+    const SlotSet c0 = ~bu[b] & ~lu[l] & mask;
+    int slot = -1;
+    if (c0.any()) {
+      slot = (int)c0._Find_first();
+    } else {
+      const SlotSet fa = ~lu[l] & mask, fc = ~bu[b] & mask;
+      for (int a = (int)fa._Find_first(); a < D && slot < 0; a = (int)fa._Find_next(a))
+        for (int c = (int)fc._Find_first(); c < D && slot < 0; c = (int)fc._Find_next(c)) {
+          // from bank b along its a-edge, then that lane's c-edge, ..: it cannot reach l
+          // (free at a) nor return to b (free at c)
+          path.clear();
+          int node = b, col = a;
+          bool at_bank = true, ok = true;
+          for (;;) {
+            const int nx = at_bank ? bank_at[(size_t)node * D + col]
+                                   : lane_at[(size_t)node * D + col];
+            if (nx == -1) break;
+            if (nx == -2) {
+              ok = false;
+              break;
+            }
+            path.push_back(at_bank ? std::make_pair(nx, node) : std::make_pair(node, nx));
+            node = nx;
+            at_bank = !at_bank;
+            col = col == a ? c : a;
+          }
+          if (!ok) continue;
+          pv.resize(path.size());
+          for (size_t i = 0; i < path.size(); ++i) {
+            const int k0 = i % 2 ? c : a, pl = path[i].first, pb = path[i].second;
+            pv[i] = sval[(size_t)pl * D + k0];
+            lane_at[(size_t)pl * D + k0] = bank_at[(size_t)pb * D + k0] = -1;
+            sval[(size_t)pl * D + k0] = -1;
+            lu[pl].reset(k0);
+            bu[pb].reset(k0);
+          }
+          for (size_t i = 0; i < path.size(); ++i) {
+            const int k1 = i % 2 ? a : c, pl = path[i].first, pb = path[i].second;
+            lane_at[(size_t)pl * D + k1] = pb;
+            bank_at[(size_t)pb * D + k1] = pl;
+            sval[(size_t)pl * D + k1] = pv[i];
+            lu[pl].set(k1);
+            bu[pb].set(k1);
+          }
+          slot = a;
+        }
+      if (slot < 0) return -1;
+    }
+    lu[l].set(slot);
+    bu[b].set(slot);
+    lane_at[(size_t)l * D + slot] = b;
+    bank_at[(size_t)b * D + slot] = l;
+    sval[(size_t)l * D + slot] = key(it.v, it.r);
+  }
+  int used = 0;
+  slot_of.assign(G, {});
+  std::vector<std::pair<int, int>> ord;  // (entry, position in the lane)
+  for (int l = 0; l < G; ++l) {
+    for (int k = 0; k < D; ++k)
+      if (sval[(size_t)l * D + k] >= 0) got[l].push_back({sval[(size_t)l * D + k], k});
+    std::sort(got[l].begin(), got[l].end());
+    ord.clear();
+    for (int r = 0; r < (int)entries[l].size(); ++r) ord.push_back({entries[l][r], r});
+    std::sort(ord.begin(), ord.end());
+    slot_of[l].assign(entries[l].size(), -1);
+    int occ = 0;
+    for (size_t i = 0; i < ord.size(); ++i) {
+      occ = (i > 0 && ord[i - 1].first == ord[i].first) ? occ + 1 : 0;
+      const auto hit = std::lower_bound(got[l].begin(), got[l].end(),
+                                        std::make_pair(key(ord[i].first, occ), -1));
+      if (hit == got[l].end() || hit->first != key(ord[i].first, occ)) return -1;
+      slot_of[l][ord[i].second] = hit->second;
+      used = std::max(used, hit->second + 1);
+    }
+  }
+  return used;
+}
+
+int schedule_bcast(const std::vector<std::vector<int>>& entries, int nbanks, int limit,
+                   std::vector<std::vector<int>>& slot_of) {
+  const int G = (int)entries.size();
+  std::vector<std::pair<int, int>> vl;
+  int lane = 0;
+  for (int l = 0; l < G; ++l) {
+    lane = std::max(lane, (int)entries[l].size());
+    for (int v : entries[l]) vl.push_back({v, l});
+  }
+  if (vl.empty()) return -1;
+  std::sort(vl.begin(), vl.end());
+  std::vector<int> bb(nbanks, 0);
+  for (size_t i = 0; i < vl.size();) {
+    size_t j = i;
+    int rmax = 0;
+    while (j < vl.size() && vl[j].first == vl[i].first) {
+      size_t b = j;
+      while (b < vl.size() && vl[b] == vl[j]) ++b;
+      rmax = std::max(rmax, (int)(b - j));
+      j = b;
+    }
+    bb[vl[i].first % nbanks] += rmax;
+    i = j;
+  }
+  const int bound = std::max(lane, *std::max_element(bb.begin(), bb.end()));
+  for (int D = bound; D < limit && D <= (int)SlotSet().size(); ++D) {
+    const int u = schedule_bcast_try(entries, nbanks, D, vl, slot_of);
+    if (u >= 0) return u;
+  }
+  return -1;
+}
+
+// The slot schedule of one lane group: schedule_group's exact colouring, replaced by the
+// broadcast-aware one when that lowers the slot count past a multiple of 4 (a wave's slots
+// per part are rounded up to 4; a smaller saving changes no instantiation).
+int schedule_slots(const std::vector<std::vector<int>>& entries, int nbanks, bool bcast,
+                   std::vector<std::vector<int>>& slot_of) {
+  const int ns = schedule_group(entries, nbanks, slot_of);
+  if (!bcast || nbanks == 0 || ns <= 4) return ns;
+  std::vector<std::vector<int>> so;
+  const int nb = schedule_bcast(entries, nbanks, (int)ceil_div(ns, 4) * 4 - 3, so);
+  if (nb < 0) return ns;
+  slot_of.swap(so);
+  return nb;
 }
 
 // Per-tile LDS layout of the table parts (32-bank model, whole-frame waves, <= 2 parts).
@@ -1291,6 +1526,7 @@ extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
       e->regroup_iters = (int)value;
       return kOk;
     case 7: e->layout = value != 0; return kOk;
+    case 8: e->bcast = value != 0; return kOk;
     default: return fail(kErrSize, "rsp_emc_set_option: unknown key");
   }
 }
@@ -1422,7 +1658,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
           }
         }
         std::vector<std::vector<int>> slot_of;
-        const int ns = schedule_group(ent, NB, slot_of);
+        const int ns = schedule_slots(ent, NB, e->bcast, slot_of);
         ns_max = std::max(ns_max, ns);
         for (int l = 0; l < G; ++l) {
           auto& pl = wv.placed[p][g0 + l];
@@ -1510,7 +1746,7 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
             for (int v : wv.placed[p][g0 + l])
               if (v >= 0) ent[l].push_back(T.map(p, v));
           std::vector<std::vector<int>> slot_of;
-          const int ns = schedule_group(ent, NB, slot_of);
+          const int ns = schedule_slots(ent, NB, e->bcast, slot_of);
           ns_max = std::max(ns_max, ns);
           for (int l = 0; l < G; ++l) {
             auto& pl = out.placed[p][g0 + l];
@@ -1891,11 +2127,17 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
 // Host-side check of the photon slot scheduler and the table layouts (no device calls):
 // `trials` random lane x bank groups (32 and 64 banks, uniform and clustered entries)
 // through schedule_group, and random tiles (1-2 parts, 8/12/16 waves) through tile_layout.
-// out[0] groups, out[1] conflicts (a lane or bank twice in one slot, an entry without a
-// valid slot), out[2] groups whose slot count is not the maximum degree, out[3] layouts,
+// Each group is scheduled twice: by schedule_group and by schedule_slots with broadcast
+// (one entry per bank and slot, shared by any number of lanes).
+// out[0] groups, out[1] conflicts (a lane twice in one slot, a bank twice in one slot --
+// with broadcast: two different entries of a bank in one slot --, an entry without a valid
+// slot), out[2] groups whose exact slot count is not the maximum degree or whose broadcast
+// schedule takes more slots than the exact one, out[3] layouts,
 // out[4] layout faults (an LDS row that is not a permutation of its table row, sigma and
 // inv not inverse, a direct-to-LDS source offset outside its chunk or not landing where
-// map() puts the entry).
+// map() puts the entry), out[5] slots the broadcast schedules saved in total, out[6] the
+// broadcast slot count of a fixed half-wave whose lanes share five entries of one bank
+// (6 is optimal).
 extern "C" int rsp_emc_schedule_check(int64_t trials, uint64_t seed, int64_t* out) {
   if (!out) return fail(kErrNull, "rsp_emc_schedule_check: null pointer");
   if (trials < 0) return fail(kErrSize, "rsp_emc_schedule_check: trials < 0");
@@ -1907,18 +2149,22 @@ extern "C" int rsp_emc_schedule_check(int64_t trials, uint64_t seed, int64_t* ou
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return (z ^ (z >> 31)) % n;
   };
-  for (int i = 0; i < 5; ++i) out[i] = 0;
+  for (int i = 0; i < 7; ++i) out[i] = 0;
   for (int64_t it = 0; it < trials; ++it) {
     const int NB = it % 3 == 0 ? 64 : 32, G = NB;
     const int mean = 1 + (int)rnd(40);
-    const bool clustered = it % 5 == 0;
+    // entries: uniform, clustered in one bank, or drawn half of the time from 48 shared
+    // "bright" entries (lanes then hold common entries, and repeats within a lane)
+    const int kind = it % 5 == 0 ? 1 : it % 5 == 1 ? 2 : 0;
     std::vector<std::vector<int>> ent(G);
     for (int l = 0; l < G; ++l) {
       const int n = (int)rnd(2 * mean);
       for (int k = 0; k < n; ++k)
-        ent[l].push_back(clustered ? (int)rnd(7) * 32 : (int)rnd(kPartMax));
+        ent[l].push_back(kind == 1 ? (int)rnd(7) * 32
+                         : kind == 2 && rnd(2) ? (int)rnd(48) * 37 % kPartMax
+                                               : (int)rnd(kPartMax));
     }
-    std::vector<std::vector<int>> so;
+    std::vector<std::vector<int>> so, sb;
     const int ns = schedule_group(ent, NB, so);
     int D = 0;
     std::vector<int> bd(NB, 0);
@@ -1939,7 +2185,48 @@ extern "C" int rsp_emc_schedule_check(int64_t trials, uint64_t seed, int64_t* ou
         out[1] += lu[(size_t)l * (ns + 1) + k]++ > 0;
         out[1] += bu[(size_t)(ent[l][r] % NB) * (ns + 1) + k]++ > 0;
       }
+    // the broadcast-aware schedule: one slot per lane entry, and in each (bank, slot) one
+    // entry, whatever the number of lanes reading it; never more slots than the above
+    const int nb = schedule_slots(ent, NB, true, sb);
+    out[2] += nb > ns;
+    std::vector<int> lv((size_t)G * (nb + 1), -1), bv((size_t)NB * (nb + 1), -1);
+    for (int l = 0; l < G; ++l)
+      for (size_t r = 0; r < ent[l].size(); ++r) {
+        const int k = (l < (int)sb.size() && r < sb[l].size()) ? sb[l][r] : -1;
+        if (k < 0 || k >= nb) {
+          ++out[1];
+          continue;
+        }
+        out[1] += lv[(size_t)l * (nb + 1) + k] >= 0;
+        lv[(size_t)l * (nb + 1) + k] = ent[l][r];
+        int& b = bv[(size_t)(ent[l][r] % NB) * (nb + 1) + k];
+        out[1] += b >= 0 && b != ent[l][r];
+        b = ent[l][r];
+      }
+    out[5] += ns - nb;
     ++out[0];
+  }
+  {
+    // every lane of a half-wave holds the same five entries of bank 0 and one of its own:
+    // the exact schedule needs 161 slots (bank 0's degree), the broadcast one 6
+    std::vector<std::vector<int>> ent(32), sb;
+    for (int l = 0; l < 32; ++l) ent[l] = {0, 32, 64, 96, 128, 200 + l};
+    const int nb = schedule_slots(ent, 32, true, sb);
+    std::vector<int> bv((size_t)32 * 256, -1);
+    for (int l = 0; l < 32 && nb > 0; ++l) {
+      std::vector<char> lu(nb, 0);
+      for (size_t r = 0; r < ent[l].size(); ++r) {
+        const int k = sb[l][r];
+        if (k < 0 || k >= nb || lu[k]++) {
+          ++out[1];
+          continue;
+        }
+        int& b = bv[(size_t)(ent[l][r] % 32) * 256 + k];
+        out[1] += b >= 0 && b != ent[l][r];
+        b = ent[l][r];
+      }
+    }
+    out[6] = nb;
   }
   const int64_t ntiles = std::max<int64_t>(1, trials / 500);
   for (int64_t it = 0; it < ntiles; ++it)

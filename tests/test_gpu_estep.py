@@ -144,7 +144,8 @@ def test_estep_table_layout_does_not_change_result(size, parts):
     """Option 7 (per-tile LDS layout of the table parts) changes only the slot layout and
     the table loads' addresses: argmax and FP64 log-likelihood are identical with and
     without it, on frames numerous enough for full workgroup tiles (185^2: two table
-    parts; 128^2: one), and the layout lowers the gather's slot count."""
+    parts; 128^2: one), and the layout lowers the gather's slot count (with the exact,
+    distinct-bank schedule: broadcast-aware slots can already sit at the lane floor)."""
     import types
     import torch
     from refspi_amd import synth
@@ -161,7 +162,7 @@ def test_estep_table_layout_does_not_change_result(size, parts):
     out = []
     for lay in (0, 1):
         emc = EMC(dataset=Dataset(frames=fr), size=cfg.size, num_rot=24, model=model, seed=1,
-                  save=False, table_layout=lay, **grid)
+                  save=False, table_layout=lay, shared_slots=0, **grid)
         emc._calculate_prob(torch.as_tensor(emc.model, device='cuda'))
         emc._get_rmax()
         out.append((emc.rmax.copy(), emc.maxll.copy(), emc.estep_timing(), emc.estep_stats()))
@@ -169,6 +170,38 @@ def test_estep_table_layout_does_not_change_result(size, parts):
     print('\nLDS read bytes per hypothesis: identity %.0f, layout %.0f' %
           (out[0][2]['lds_read_bytes_per_hyp'], out[1][2]['lds_read_bytes_per_hyp']))
     assert out[1][3]['table_parts'] == parts
+    assert np.array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert out[1][2]['lds_read_bytes_per_hyp'] < out[0][2]['lds_read_bytes_per_hyp']
+
+
+def test_estep_shared_slots_do_not_change_result():
+    """Option 8 (broadcast-aware photon slots: lanes that read the same table entry share
+    its bank in a slot) changes only the slot layout: argmax and FP64 log-likelihood are
+    identical with and without it at 185^2 (two table parts, per-tile layouts on), and it
+    lowers the gather's LDS reads."""
+    import types
+    import torch
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.emc import EMC
+    cfg = synth.SynthConfig(**{**synth.CONFIG_185.__dict__, 'num_data': 6000, 'seed': 13})
+    d = synth.make_frames_torch(cfg, device='cuda')
+    fr = types.SimpleNamespace(num_pix=d['num_pix'], bg=None, **{
+        k: d[k].cpu().numpy().astype('i4')
+        for k in ('ones', 'multi', 'place_ones', 'place_multi', 'count_multi')})
+    model = synth.model_scales(cfg)[0] * synth.object_ft(cfg)
+    grid = dict(shiftx=(-1., 1., 3), shifty=(-1., 1., 3), sphere_dia=(7., 9., 2))
+    out = []
+    for shared in (0, 1):
+        emc = EMC(dataset=Dataset(frames=fr), size=cfg.size, num_rot=24, model=model, seed=1,
+                  save=False, shared_slots=shared, **grid)
+        emc._calculate_prob(torch.as_tensor(emc.model, device='cuda'))
+        emc._get_rmax()
+        out.append((emc.rmax.copy(), emc.maxll.copy(), emc.estep_timing()))
+        del emc
+    print('\nLDS read bytes per hypothesis: distinct banks %.0f, shared slots %.0f' %
+          (out[0][2]['lds_read_bytes_per_hyp'], out[1][2]['lds_read_bytes_per_hyp']))
     assert np.array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
     assert out[1][2]['lds_read_bytes_per_hyp'] < out[0][2]['lds_read_bytes_per_hyp']
