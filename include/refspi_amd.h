@@ -116,7 +116,11 @@ int rsp_emc_set_frames(rsp_emc* plan, int64_t num_data, const int32_t* ones,
  *       result.
  *   8 = broadcast-aware photon slots (1 = on, the default; 0 = off; set before frames):
  *       lanes that read the same table entry may share a bank in one slot (one LDS access
- *       serves them).  Only the slot layout changes, never the result. */
+ *       serves them).  Only the slot layout changes, never the result.
+ *   9 = frames per lane of whole-frame waves (2, the default: each slot register holds
+ *       the 16-bit LDS addresses of two frames, so a workgroup covers twice the frames per
+ *       table load; 1 = one frame per lane; set before frames).  Only which lanes and
+ *       registers hold a frame changes, never the result. */
 int rsp_emc_set_option(rsp_emc* plan, int key, int64_t value);
 
 /* Run the E-step for `model_c` (device, npix complex) with per-frame `scales` (device,

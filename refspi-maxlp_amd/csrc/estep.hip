@@ -373,6 +373,84 @@ __device__ __forceinline__ void gather_range(const char* lds, const uint32_t (&a
   }
 }
 
+// Two frames per lane: each slot register packs the LDS byte addresses of frame 0 (low
+// 16 bits) and frame 1 (high 16 bits).  One step walks the virtual slots k = 0 .. 2S-1 of
+// part PP: frame k / S, register PP*S + k % S, with the same software pipeline as above
+// (one extraction per read); blocks do not straddle the frames, a frame's last block is
+// added unpaired, and frame f accumulates in acc[2f], acc[2f + 1].
+template <int S, int K>
+constexpr int block_len2() {
+  return S - K % S >= 8 ? 8 : S - K % S;
+}
+
+// ext = {0xffff, 16} as opaque per-step values: the extractions cannot be hoisted out of
+// the hypothesis loop (which would hold two address registers per slot register)
+template <int S, int PP, int K, int N, int P, bool REB>
+__device__ __forceinline__ void load_block2(const char* lds, const uint32_t (&a)[P],
+                                            float (&v)[8], uint32_t rb, const uint32_t (&ext)[2]) {
+  constexpr int f = K / S, r0 = PP * S + K % S;
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    if constexpr (REB) {
+      // third buffer: the address with bit 16 set (v_and_or_b32 / v_alignbit_b32: one
+      // operation, like the other buffers' extraction), and its base - 65536 as the
+      // immediate offset
+      const uint32_t x = f ? __builtin_amdgcn_alignbit(rb, a[r0 + u], 16u)
+                           : ((a[r0 + u] & ext[0]) | (rb << 16));
+      v[u] = *reinterpret_cast<const float*>(lds - 65536 + x);
+    } else {
+      const uint32_t x = f ? (a[r0 + u] >> ext[1]) : (a[r0 + u] & ext[0]);
+      v[u] = *reinterpret_cast<const float*>(lds + x);
+    }
+  }
+}
+
+template <int S, int PP, int K, int P, int B, bool REB, bool PEND>
+__device__ __forceinline__ void gather_pipe2(const char* lds, const uint32_t (&a)[P],
+                                             float (&v)[2][8], double (&acc)[4], uint32_t rb,
+                                             float pend, const uint32_t (&ext)[2]) {
+  if constexpr (K < 2 * S) {
+    constexpr int N = block_len2<S, K>();
+    static_assert(N == 8 || N == 4, "parts hold a multiple of 4 slots");
+    constexpr int f = K / S;
+    constexpr bool last = (K + N) % S == 0;  // the frame's last block
+    if constexpr (K + N < 2 * S)
+      load_block2<S, PP, K + N, block_len2<S, K + N>(), P, REB>(lds, a, v[B ^ 1], rb, ext);
+    const float* w = v[B];
+    float s8;
+    if constexpr (N == 8)
+      s8 = ((w[0] + w[1]) + (w[2] + w[3])) + ((w[4] + w[5]) + (w[6] + w[7]));
+    else
+      s8 = (w[0] + w[1]) + (w[2] + w[3]);
+    asm volatile("" : "+v"(s8) : : "memory");
+    constexpr int ai = 2 * f + (((K % S) >> 4) & 1);
+    if constexpr (PEND) {
+      acc[ai] += (double)(pend + s8);
+      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, false>(lds, a, v, acc, rb, 0.f, ext);
+    } else if constexpr (last) {
+      acc[ai] += (double)s8;
+      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, false>(lds, a, v, acc, rb, 0.f, ext);
+    } else {
+      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, true>(lds, a, v, acc, rb, s8, ext);
+    }
+  }
+}
+
+template <int S, int PP, int P, bool REB>
+__device__ __forceinline__ void gather_range2(const char* lds, const uint32_t (&a)[P],
+                                              double (&acc)[4]) {
+  // rb = 1 (opaque): the third buffer's addresses carry bit 16
+  uint32_t rb = 0;
+  if constexpr (REB) asm volatile("" : "=s"(rb) : "0"(1u));
+  // the mask in a VGPR: v_and_or_b32 (VOP3) takes one scalar operand on gfx9
+  uint32_t ext[2];
+  asm volatile("" : "=v"(ext[0]) : "0"(0xffffu));
+  asm volatile("" : "=s"(ext[1]) : "0"(16u));
+  float v[2][8];
+  load_block2<S, PP, 0, block_len2<S, 0>(), P, REB>(lds, a, v[0], rb, ext);
+  gather_pipe2<S, PP, 0, P, 0, REB, false>(lds, a, v, acc, rb, 0.f, ext);
+}
+
 template <int K0, int P>
 __device__ __forceinline__ void load_slots(uint32_t (&a)[P], const uint32_t* __restrict__ src) {
   if constexpr (K0 < P) {
@@ -403,12 +481,13 @@ __device__ __forceinline__ float* part_buf() {
 // q: wait until its part is in LDS (this wave's loads via vmcnt, every wave's via the
 // barrier), start the loads of step q+2 into the buffer freed by step q-1, then gather the
 // part's S slots.  Template: S slots per part and lane, NP parts, NW waves, L loads/wave.
-template <int S, int NP, int NW, int L, int X>
+template <int S, int NP, int NW, int L, int X, int F>
 struct GatherStep {
   static constexpr int P = S * NP;
+  static_assert(F == 1 || (F == 2 && X == 1), "two frames per lane: whole frames only");
   const float* tables;
   int TS, hb, h0, h1, nq, R, wid, lane;
-  double scale;
+  double scale[F];  // frame f of the lane (F = 2: low / high halves of the slot registers)
   // this lane's byte offset within each 1 KB chunk for parts 0 and 1 (the tile's table
   // layout: 16 * lane for the identity)
   unsigned off0, off1;
@@ -450,7 +529,7 @@ struct GatherStep {
   template <int QS>
   __device__ __forceinline__ void step(int g, const uint32_t (&a)[P],
                                        const double* __restrict__ init, double (&acc)[4],
-                                       Top3& top) const {
+                                       Top3 (&top)[F]) const {
     constexpr int PP = QS % NP;
     const int q = g * 3 * NP + QS;
     const int h = h0 + g * 3 + QS / NP;
@@ -467,8 +546,24 @@ struct GatherStep {
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
     issue<(QS + 2) % NP>(q + 2, part_buf<(QS + 2) % 3>());
+    if constexpr (F == 2) {
+      if constexpr (PP == 0) {
+        const double in = init[min(h, h1 - 1) / R];
+        acc[0] = in * scale[0];
+        acc[2] = in * scale[1];
+        acc[1] = acc[3] = 0.;
+      }
+      gather_range2<S, PP, P, QS % 3 == 2>(reinterpret_cast<const char*>(part_buf<QS % 3>()),
+                                           a, acc);
+      if constexpr (PP == NP - 1) {
+        const double t0 = acc[0] + acc[1], t1 = acc[2] + acc[3];
+        if (h < h1 && t0 >= top[0].v3) top3_insert(top[0], t0, h);
+        if (h < h1 && t1 >= top[1].v3) top3_insert(top[1], t1, h);
+      }
+      return;
+    }
     if constexpr (PP == 0) {
-      acc[0] = init[min(h, h1 - 1) / R] * scale;
+      acc[0] = init[min(h, h1 - 1) / R] * scale[0];
       acc[1] = acc[2] = acc[3] = 0.;
     }
     gather_range<PP * S, PP * S + S, P, QS % 3 == 2>(
@@ -483,9 +578,9 @@ struct GatherStep {
       // and a wave with no lane above its v3 skips the insertion (not for 96 slots: the
       // branch's registers would push slot addresses to scratch)
       if constexpr (P < 96) {
-        if (h < h1 && tot >= top.v3) top3_insert(top, tot, h);
+        if (h < h1 && tot >= top[0].v3) top3_insert(top[0], tot, h);
       } else {
-        if (h < h1) top3_insert(top, tot, h);
+        if (h < h1) top3_insert(top[0], tot, h);
       }
     }
   }
@@ -493,7 +588,7 @@ struct GatherStep {
   template <int QS>
   __device__ __forceinline__ void group(int g, const uint32_t (&a)[P],
                                         const double* __restrict__ init, double (&acc)[4],
-                                        Top3& top) const {
+                                        Top3 (&top)[F]) const {
     if constexpr (QS < 3 * NP) {
       step<QS>(g, a, init, acc, top);
       group<QS + 1>(g, a, init, acc, top);
@@ -501,7 +596,7 @@ struct GatherStep {
   }
 };
 
-template <int S, int NP, int NW, int L, int X>
+template <int S, int NP, int NW, int L, int X, int F>
 __global__ __launch_bounds__(64 * NW) void k_gather(
     const float* __restrict__ tables, int TS, int hb, int nb, int nseg, int n_tiles, int gw0,
     const uint32_t* __restrict__ slots, const double* __restrict__ scale_sorted,
@@ -523,21 +618,24 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   // scalar too (no address registers per load)
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int lw = tile * NW + wid;  // wave within this slot class
-  const int gw = gw0 + lw;         // global wave
-  const int j = gw * 64 + lane;
+  // frames (global sorted order) of this lane: F per lane, the waves gw0 + F*lw + f
+  const int j = (gw0 + F * lw) * 64 + lane;
   const int h0 = hb + (int)((int64_t)seg * nb / nseg);
   const int h1 = hb + (int)((int64_t)(seg + 1) * nb / nseg);
 
   uint32_t a[P];
   load_slots<0, P>(a, slots + (int64_t)lw * P * 64 + lane);
-  Top3 top = top3_empty();
+  Top3 top[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) top[f] = top3_empty();
   // every wave issues exactly L direct-to-LDS loads per step; two steps are in flight
   // ahead of the gather
   // lay[p * Npad + j]: the lane's chunk offset for part p (parts 0 and 1; more parts keep
-  // the identity)
-  const GatherStep<S, NP, NW, L, X> st{tables, TS, hb, h0, h1, (h1 - h0) * NP, R, wid, lane,
-                                    scale_sorted[j], NP <= 2 ? lay[j] : 0u,
-                                    NP == 2 ? lay[Npad + j] : 0u};
+  // the identity), at the lane's first frame
+  GatherStep<S, NP, NW, L, X, F> st{tables, TS, hb, h0, h1, (h1 - h0) * NP, R, wid, lane,
+                                    {}, NP <= 2 ? lay[j] : 0u, NP == 2 ? lay[Npad + j] : 0u};
+#pragma unroll
+  for (int f = 0; f < F; ++f) st.scale[f] = scale_sorted[j + 64 * f];
   st.template issue<0>(0, g_buf0);
   st.template issue<1 % NP>(1, g_buf1);
   double acc[4];
@@ -545,7 +643,8 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   for (int g = 0; g < ngroups; ++g) st.template group<0>(g, a, init, acc, top);
   // no direct-to-LDS write may still be in flight when the workgroup's LDS is released
   __builtin_amdgcn_s_waitcnt(0xF70);
-  partials[(int64_t)seg * Npad + j] = top;
+#pragma unroll
+  for (int f = 0; f < F; ++f) partials[(int64_t)seg * Npad + j + 64 * f] = top[f];
 }
 
 // per-class segment counts for the merge (waves [gw_end[c-1], gw_end[c]) use nseg[c])
@@ -799,6 +898,7 @@ struct rsp_emc {
   int regroup_iters = 4000;  // local-search swaps per window of 8 lane groups (0: off)
   int layout = 1;            // per-tile LDS layout of the table parts (0: identity)
   int bcast = 1;             // broadcast-aware slot schedule (0: distinct banks per slot)
+  int fpl = 2;               // frames per lane of whole-frame waves (1 or 2)
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   struct Class {
     int cfg, n_tiles, gw0;
@@ -1383,21 +1483,27 @@ constexpr int nw_of(int P, int NP, int X) {
   const int p16 = X == 1 ? 96 : 80;
   return P <= p16 ? 16 : (NP <= 2 ? (P <= 120 ? 12 : 8) : (P <= 96 ? 12 : 8));
 }
+// two frames per lane: P packed slot registers + ~52 working VGPRs within 512 / (NW / 4)
+constexpr int nw2_of(int P) { return P <= 72 ? 16 : (P <= 112 ? 12 : 8); }
 constexpr int l_of(int P, int NP, int X) {
   return (kChunks + nw_of(P, NP, X) - 1) / nw_of(P, NP, X);
 }
 
 struct GatherCfg {
   int S, NP, NW, X;
+  int F;  // frames per lane (2: slot registers hold two 16-bit addresses)
   GatherFn fn;
 };
 
 #define RSP_CFG(S, NP) \
-  {S, NP, nw_of(S * NP, NP, 1), 1,  \
-   &k_gather<S, NP, nw_of(S * NP, NP, 1), l_of(S * NP, NP, 1), 1>}
+  {S, NP, nw_of(S * NP, NP, 1), 1, 1,  \
+   &k_gather<S, NP, nw_of(S * NP, NP, 1), l_of(S * NP, NP, 1), 1, 1>}
 #define RSP_CFGX(S, NP, X) \
-  {S, NP, nw_of(S * NP, NP, X), X,  \
-   &k_gather<S, NP, nw_of(S * NP, NP, X), l_of(S * NP, NP, X), X>}
+  {S, NP, nw_of(S * NP, NP, X), X, 1,  \
+   &k_gather<S, NP, nw_of(S * NP, NP, X), l_of(S * NP, NP, X), X, 1>}
+#define RSP_CFG2(S, NP) \
+  {S, NP, nw2_of(S * NP), 1, 2,  \
+   &k_gather<S, NP, nw2_of(S * NP), (kChunks + nw2_of(S * NP) - 1) / nw2_of(S * NP), 1, 2>}
 const GatherCfg kGatherCfgs[] = {
     RSP_CFG(16, 1), RSP_CFG(32, 1), RSP_CFG(48, 1), RSP_CFG(64, 1), RSP_CFG(80, 1),
     RSP_CFG(96, 1), RSP_CFG(112, 1), RSP_CFG(128, 1), RSP_CFG(160, 1),
@@ -1416,22 +1522,31 @@ const GatherCfg kGatherCfgs[] = {
     RSP_CFGX(24, 3, 16), RSP_CFGX(24, 3, 32), RSP_CFGX(24, 3, 64),
     RSP_CFGX(16, 4, 2), RSP_CFGX(16, 4, 4), RSP_CFGX(16, 4, 8),
     RSP_CFGX(16, 4, 16), RSP_CFGX(16, 4, 32), RSP_CFGX(16, 4, 64),
+    // two frames per lane (whole frames)
+    RSP_CFG2(16, 1), RSP_CFG2(32, 1), RSP_CFG2(48, 1), RSP_CFG2(64, 1),
+    RSP_CFG2(8, 2), RSP_CFG2(16, 2), RSP_CFG2(24, 2), RSP_CFG2(28, 2), RSP_CFG2(32, 2),
+    RSP_CFG2(36, 2), RSP_CFG2(40, 2), RSP_CFG2(44, 2), RSP_CFG2(48, 2),
 };
 #undef RSP_CFG
 #undef RSP_CFGX
+#undef RSP_CFG2
 constexpr int kNumCfg = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
 
-// smallest instantiation with NP parts and lane split X whose part capacity holds `need`
-int choose_cfg(int NP, int X, int need) {
+// smallest instantiation with NP parts, lane split X and F frames per lane whose part
+// capacity holds `need`
+int choose_cfg(int NP, int X, int need, int F = 1) {
   for (int i = 0; i < kNumCfg; ++i)
-    if (kGatherCfgs[i].NP == NP && kGatherCfgs[i].X == X && need <= kGatherCfgs[i].S) return i;
+    if (kGatherCfgs[i].NP == NP && kGatherCfgs[i].X == X && kGatherCfgs[i].F == F &&
+        need <= kGatherCfgs[i].S)
+      return i;
   return -1;
 }
 
 int max_slots(int NP, int X) {
   int m = 0;
   for (int i = 0; i < kNumCfg; ++i)
-    if (kGatherCfgs[i].NP == NP && kGatherCfgs[i].X == X) m = std::max(m, kGatherCfgs[i].S);
+    if (kGatherCfgs[i].NP == NP && kGatherCfgs[i].X == X && kGatherCfgs[i].F == 1)
+      m = std::max(m, kGatherCfgs[i].S);
   return m;
 }
 
@@ -1527,6 +1642,10 @@ extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
       return kOk;
     case 7: e->layout = value != 0; return kOk;
     case 8: e->bcast = value != 0; return kOk;
+    case 9:
+      if (value != 1 && value != 2) return fail(kErrSize, "frames per lane must be 1 or 2");
+      e->fpl = (int)value;
+      return kOk;
     default: return fail(kErrSize, "rsp_emc_set_option: unknown key");
   }
 }
@@ -1716,10 +1835,16 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   std::stable_sort(worder.begin(), worder.end(), [&](int64_t a, int64_t b) {
     return waves[a].X != waves[b].X ? waves[a].X > waves[b].X : waves[a].need > waves[b].need;
   });
+  // whole-frame waves go two to a hardware wave (option 9) where an instantiation holds
+  // their slots; split frames and heavier waves one per hardware wave
+  auto cfg_of = [&](int X, int need) {
+    const int c2 = X == 1 && e->fpl == 2 ? choose_cfg(e->NP, 1, need, 2) : -1;
+    return c2 >= 0 ? c2 : choose_cfg(e->NP, X, need);
+  };
   std::vector<int> wcfg(nw0);
   for (int64_t i = 0; i < nw0; ++i) {
     const Wave& wv = waves[worder[i]];
-    const int c = choose_cfg(e->NP, wv.X, std::max(wv.need, 8));
+    const int c = cfg_of(wv.X, std::max(wv.need, 8));
     if (c < 0) return fail(kErrUnsupported, "rsp_emc_set_frames: a wave needs too many slots");
     wcfg[i] = c;
   }
@@ -1789,8 +1914,9 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
         return lfl[x] != lfl[y] ? lfl[x] > lfl[y] : waves[worder[x]].need > waves[worder[y]].need;
       });
       for (size_t t0 = 0; t0 < cand.size();) {
-        const int c0 = choose_cfg(e->NP, 1, std::max(lfl[cand[t0]], 8));
-        const int NWt = kGatherCfgs[c0].NW;
+        // NWt: frames' waves per workgroup (hardware waves x frames per lane)
+        const int c0 = cfg_of(1, std::max(lfl[cand[t0]], 8));
+        const int NWh = kGatherCfgs[c0].NW, Ft = kGatherCfgs[c0].F, NWt = NWh * Ft;
         if (t0 + NWt > cand.size()) break;
         std::vector<const Placed*> ps;
         int64_t old_cost = 0;
@@ -1799,22 +1925,22 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
           old_cost += kGatherCfgs[wcfg[cand[t0 + k]]].S;
         }
         TileLayout T;
-        tile_layout(e->NP, NWt, ps, T);
+        tile_layout(e->NP, NWh, ps, T);
         std::vector<Wave> nws(NWt);
         int need = 0;
         for (int k = 0; k < NWt; ++k) {
           resched(waves[worder[cand[t0 + k]]], T, nws[k]);
           need = std::max(need, nws[k].need);
         }
-        const int c2 = choose_cfg(e->NP, 1, std::max(need, 8));
-        if (c2 >= 0 && kGatherCfgs[c2].NW == NWt &&
+        const int c2 = cfg_of(1, std::max(need, 8));
+        if (c2 >= 0 && kGatherCfgs[c2].NW == NWh && kGatherCfgs[c2].F == Ft &&
             (int64_t)NWt * kGatherCfgs[c2].S < old_cost) {
           Unit u{1, c2, true, {}};
           for (int k = 0; k < NWt; ++k) {
             const int64_t wi = worder[cand[t0 + k]];
             waves[wi] = std::move(nws[k]);
             wlay[wi] = (int)layouts.size();
-            wwid[wi] = k;
+            wwid[wi] = k / Ft;  // the hardware wave that loads its chunks
             in_tile[cand[t0 + k]] = 1;
             u.w.push_back(wi);
           }
@@ -1826,16 +1952,20 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
     }
     for (int64_t i = 0; i < nw0; ++i)
       if (!in_tile[i]) units.push_back(Unit{waves[worder[i]].X, wcfg[i], false, {worder[i]}});
-    // lane split descending, then capacity descending (the order above), tiles first
+    // lane split descending, frames per lane descending, then capacity descending (the
+    // order above), tiles first
     std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& b) {
       if (a.X != b.X) return a.X > b.X;
+      if (kGatherCfgs[a.cfg].F != kGatherCfgs[b.cfg].F)
+        return kGatherCfgs[a.cfg].F > kGatherCfgs[b.cfg].F;
       if (a.cfg != b.cfg) return a.cfg > b.cfg;
       return a.tile && !b.tile;
     });
     // Runs of one instantiation (the classes): tiles, then single waves.  The last
     // workgroup of a class would be completed with empty waves, each reading its slots as
     // zero pads; it takes the neediest single waves of the following classes with the
-    // same lane split instead (their capacity is smaller, so they fit).
+    // same lane split and frames per lane instead (their capacity is smaller, so they
+    // fit).
     struct Run {
       int X, cfg;
       std::vector<int64_t> tiles, singles;
@@ -1847,10 +1977,12 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
       dst.insert(dst.end(), u.w.begin(), u.w.end());
     }
     for (size_t a = 0; a < runs.size(); ++a) {
-      const int NWa = kGatherCfgs[runs[a].cfg].NW;
+      const int Fa = kGatherCfgs[runs[a].cfg].F, NWa = kGatherCfgs[runs[a].cfg].NW * Fa;
       const int64_t n = (int64_t)(runs[a].tiles.size() + runs[a].singles.size());
       int64_t short_by = (NWa - n % NWa) % NWa;
-      for (size_t b = a + 1; short_by > 0 && b < runs.size() && runs[b].X == runs[a].X; ++b) {
+      for (size_t b = a + 1; short_by > 0 && b < runs.size() && runs[b].X == runs[a].X &&
+                             kGatherCfgs[runs[b].cfg].F == Fa;
+           ++b) {
         auto& src = runs[b].singles;
         const int64_t take = std::min<int64_t>(short_by, (int64_t)src.size());
         runs[a].singles.insert(runs[a].singles.end(), src.begin(), src.begin() + take);
@@ -1877,10 +2009,10 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
     const GatherCfg& g = kGatherCfgs[c.cfg];
     c.w_src = i;
     c.n_src = i1 - i;
-    c.n_tiles = (int)ceil_div(c.n_src, g.NW);
+    c.n_tiles = (int)ceil_div(c.n_src, g.NW * g.F);
     c.gw0 = (int)gw_total;
     c.slot_off = slot_total;
-    gw_total += (int64_t)c.n_tiles * g.NW;
+    gw_total += (int64_t)c.n_tiles * g.NW * g.F;  // waves of frames
     slot_total += (int64_t)c.n_tiles * g.NW * g.S * g.NP * 64;
     e->classes.push_back(c);
     i = i1;
@@ -1894,17 +2026,23 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
   for (const auto& c : e->classes) {
     const GatherCfg& g = kGatherCfgs[c.cfg];
     const int P = g.S * g.NP;
-    for (int64_t lw = 0; lw < (int64_t)c.n_tiles * g.NW; ++lw) {
-      const int64_t gw = c.gw0 + lw;
-      const bool real = lw < c.n_src;
-      const Wave* wv = real ? &waves[worder[c.w_src + lw]] : nullptr;
+    // waves of frames lwf; hardware wave lw = lwf / F holds them in the low (f = 0) and
+    // high (f = 1) halves of its slot registers when F = 2
+    for (int64_t lwf = 0; lwf < (int64_t)c.n_tiles * g.NW * g.F; ++lwf) {
+      const int64_t lw = lwf / g.F, gw = c.gw0 + lwf;
+      const int f = (int)(lwf % g.F);
+      const bool real = lwf < c.n_src;
+      const Wave* wv = real ? &waves[worder[c.w_src + lwf]] : nullptr;
       // the first lane of each frame's group stands for the frame; the others are -1
       for (int l = 0; l < 64; ++l)
         if (real && l % wv->X == 0 && l / wv->X < wv->nf)
           perm[gw * 64 + l] = order[wv->f0 + l / wv->X];
-      if (real && wlay[worder[c.w_src + lw]] >= 0) {
-        const TileLayout& T = layouts[wlay[worder[c.w_src + lw]]];
-        const int wid = wwid[worder[c.w_src + lw]];
+      // the chunk offsets of the hardware wave's loads, read at its first frame
+      if (real && f == 0 && wlay[worder[c.w_src + lwf]] >= 0) {
+        const TileLayout& T = layouts[wlay[worder[c.w_src + lwf]]];
+        const int wid = wwid[worder[c.w_src + lwf]];
+        if (wid != (int)(lw % g.NW))
+          return fail(kErrUnsupported, "rsp_emc_set_frames: tile out of its workgroup");
         for (int p = 0; p < g.NP; ++p)
           for (int l = 0; l < 64; ++l)
             lay[(size_t)p * Npad + gw * 64 + l] =
@@ -1948,8 +2086,16 @@ extern "C" int rsp_emc_set_frames(rsp_emc* e, int64_t num_data, const int32_t* o
                   used |= 1ull << bank;
                 }
               }
-              slots[(size_t)(c.slot_off + ((int64_t)lw * P + p * g.S + k) * 64 + g0 + l)] =
-                  (uint32_t)entry * 4u;
+              uint32_t& sl =
+                  slots[(size_t)(c.slot_off + ((int64_t)lw * P + p * g.S + k) * 64 + g0 + l)];
+              if (g.F == 1) {
+                sl = (uint32_t)entry * 4u;
+              } else {
+                // two 16-bit LDS byte addresses (a part buffer is 50,176 bytes)
+                if ((uint32_t)entry * 4u > 0xffffu)
+                  return fail(kErrUnsupported, "rsp_emc_set_frames: slot address above 16 bits");
+                sl = f == 0 ? (uint32_t)entry * 4u : sl | ((uint32_t)entry * 4u << 16);
+              }
             }
           }
         }
@@ -2054,7 +2200,7 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
         const auto& c = e->classes[ci];
         const GatherCfg& gc = kGatherCfgs[c.cfg];
         const int ns = nseg_of(c, nb);
-        sm.gw_end[ci] = c.gw0 + c.n_tiles * gc.NW;
+        sm.gw_end[ci] = c.gw0 + c.n_tiles * gc.NW * gc.F;
         sm.nseg[ci] = ns;
         hipLaunchKernelGGL(gc.fn, dim3((unsigned)(ns * c.n_tiles)), dim3(64 * gc.NW), lds_bytes,
                            st, e->d_tables, e->TS, (int)hb, nb, ns, c.n_tiles, c.gw0,
@@ -2295,7 +2441,7 @@ extern "C" int rsp_emc_timing(const rsp_emc* e, double* out) {
   for (const auto& c : e->classes) {
     const GatherCfg& g = kGatherCfgs[c.cfg];
     const int L = (kChunks + g.NW - 1) / g.NW;
-    rd += (double)c.n_tiles * g.NW * 64. * g.S * g.NP * 4.;
+    rd += (double)c.n_tiles * g.NW * g.F * 64. * g.S * g.NP * 4.;
     wr += (double)c.n_tiles * g.NP * g.NW * L * 1024.;
   }
   out[4] = rd;

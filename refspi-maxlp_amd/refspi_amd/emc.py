@@ -67,7 +67,8 @@ class EMC:
     def __init__(self, config_file=None, num_streams=4, *, dataset=None, size=None,
                  num_rot=None, shiftx=None, shifty=None, sphere_dia=None, need_scaling=None,
                  output_folder=None, model=None, seed=None, device=0, save=True,
-                 bank_count=None, regroup_iters=None, table_layout=None, shared_slots=None):
+                 bank_count=None, regroup_iters=None, table_layout=None, shared_slots=None,
+                 frames_per_lane=None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('EMC needs a GPU (HIP device)')
@@ -79,6 +80,7 @@ class EMC:
         self.regroup_iters = regroup_iters
         self.table_layout = table_layout
         self.shared_slots = shared_slots
+        self.frames_per_lane = frames_per_lane
         config = configparser.ConfigParser()
         start_dir = '.'
         if config_file is not None:
@@ -182,6 +184,9 @@ class EMC:
                        'rsp_emc_set_option')
         if self.shared_slots is not None:   # lanes reading one entry share its bank slot
             _lib.check(lib.rsp_emc_set_option(h, 8, int(bool(self.shared_slots))),
+                       'rsp_emc_set_option')
+        if self.frames_per_lane is not None:   # whole frames two to a lane (2) or one
+            _lib.check(lib.rsp_emc_set_option(h, 9, int(self.frames_per_lane)),
                        'rsp_emc_set_option')
         d = self.dset
         bg = np.ascontiguousarray(d.bg, dtype='f8')

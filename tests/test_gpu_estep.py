@@ -144,8 +144,9 @@ def test_estep_table_layout_does_not_change_result(size, parts):
     """Option 7 (per-tile LDS layout of the table parts) changes only the slot layout and
     the table loads' addresses: argmax and FP64 log-likelihood are identical with and
     without it, on frames numerous enough for full workgroup tiles (185^2: two table
-    parts; 128^2: one), and the layout lowers the gather's slot count (with the exact,
-    distinct-bank schedule: broadcast-aware slots can already sit at the lane floor)."""
+    parts; 128^2: one), and the layout lowers the gather's slot count (one frame per lane
+    and the exact, distinct-bank schedule: broadcast-aware slots can already sit at the lane
+    floor, and tiles of two frames per lane are twice as many waves)."""
     import types
     import torch
     from refspi_amd import synth
@@ -162,7 +163,7 @@ def test_estep_table_layout_does_not_change_result(size, parts):
     out = []
     for lay in (0, 1):
         emc = EMC(dataset=Dataset(frames=fr), size=cfg.size, num_rot=24, model=model, seed=1,
-                  save=False, table_layout=lay, shared_slots=0, **grid)
+                  save=False, table_layout=lay, shared_slots=0, frames_per_lane=1, **grid)
         emc._calculate_prob(torch.as_tensor(emc.model, device='cuda'))
         emc._get_rmax()
         out.append((emc.rmax.copy(), emc.maxll.copy(), emc.estep_timing(), emc.estep_stats()))
@@ -205,6 +206,41 @@ def test_estep_shared_slots_do_not_change_result():
     assert np.array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
     assert out[1][2]['lds_read_bytes_per_hyp'] < out[0][2]['lds_read_bytes_per_hyp']
+
+
+@pytest.mark.parametrize('size,n', [(185, 6000), (128, 3000)])
+def test_estep_two_frames_per_lane_do_not_change_result(size, n):
+    """Option 9 (whole frames two to a lane, the slot registers holding two 16-bit LDS
+    addresses) changes only which lanes and registers hold a frame's photons: argmax and
+    FP64 log-likelihood are identical with one and two frames per lane (185^2: two table
+    parts; 128^2: one), and the table loads per hypothesis fall (a workgroup of two-frame
+    waves holds twice the frames; at these frame counts partial workgroups remain)."""
+    import types
+    import torch
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.emc import EMC
+    cfg = synth.SynthConfig(**{**synth.CONFIG_185.__dict__, 'num_data': n, 'seed': 23,
+                               'size': size})
+    d = synth.make_frames_torch(cfg, device='cuda')
+    fr = types.SimpleNamespace(num_pix=d['num_pix'], bg=None, **{
+        k: d[k].cpu().numpy().astype('i4')
+        for k in ('ones', 'multi', 'place_ones', 'place_multi', 'count_multi')})
+    model = synth.model_scales(cfg)[0] * synth.object_ft(cfg)
+    grid = dict(shiftx=(-1., 1., 3), shifty=(-1., 1., 3), sphere_dia=(7., 9., 2))
+    out = []
+    for fpl in (1, 2):
+        emc = EMC(dataset=Dataset(frames=fr), size=cfg.size, num_rot=24, model=model, seed=1,
+                  save=False, frames_per_lane=fpl, **grid)
+        emc._calculate_prob(torch.as_tensor(emc.model, device='cuda'))
+        emc._get_rmax()
+        out.append((emc.rmax.copy(), emc.maxll.copy(), emc.estep_timing()))
+        del emc
+    print('\nLDS write bytes per hypothesis (table loads): one frame per lane %.0f, two %.0f'
+          % (out[0][2]['lds_write_bytes_per_hyp'], out[1][2]['lds_write_bytes_per_hyp']))
+    assert np.array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    assert out[1][2]['lds_write_bytes_per_hyp'] < 0.8 * out[0][2]['lds_write_bytes_per_hyp']
 
 
 def test_estep_bank_models_full_parts():
