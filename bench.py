@@ -587,8 +587,12 @@ def run():
                       what='one 4-byte LDS read per scored photon per frame x hypothesis '
                            '(%.0f scored photons per step-hypothesis), / gather time' % scored,
                       peak_what='256 CUs x 128 B/clk (32 banks x 4 B, the ds_read_b32 ceiling) '
-                                'x 2.4 GHz; measured conflict-free ds_read_b32: 27 of 32 '
-                                'lanes per clock',
+                                'x 2.4 GHz; measured conflict-free ds_read_b32: 24.8-27 of 32 '
+                                'lanes per clock.  Wider reads reach more bytes (ds_read_b64 '
+                                '178, b128 233 B/clk/CU measured) but one address per lane '
+                                'and clock: a gather of single 4-byte values is bound by '
+                                'the 32 lanes per clock of this figure',
+                      lds_hw_read_peak_gbs=256 * 256 * 2.4,
                       executed_gbs=lds_exec_gbs, executed_frac=lds_exec_gbs / LDS_PEAK_GBS,
                       executed_what='every slot read (bank-schedule zero pads included) + the '
                                     'table loads\' LDS writes',
