@@ -243,8 +243,11 @@ def split_leg(args):
     # K = 2 x scored pixels (re/im interleaved), 2 M N K flops
     useful = 4. * R * N * K
     res = {}
-    for mode in ('fp64', 'fp32', 'split'):
-        plan.set_mode(mode)
+    # split: the reference rows generated inside the GEMM kernel (the default);
+    # split_stored: written to HBM as bf16 hi/lo rows and read by the GEMM
+    for mode in ('fp64', 'fp32', 'split_stored', 'split'):
+        plan.set_mode(mode.split('_')[0])
+        plan.set_ref_rows('stored' if mode == 'split_stored' else 'generated')
         plan.estep_device(model, sx, sy, dia, sc, want_ll=False, want_prob=False)  # warm-up
         torch.cuda.synchronize()
         best = None
@@ -266,17 +269,23 @@ def split_leg(args):
                          frame_rotation_evals_per_s=N * R / el)
         log('split leg %s: %.3f s %s' % (mode, el, tm))
     ll64 = res['fp64']
-    for mode in ('fp32', 'split'):
+    for mode in ('fp32', 'split_stored', 'split'):
         err = ((res[mode] - ll64).abs() / ll64.abs()).max().item()
         out[mode]['ll_max_rel_err_vs_fp64'] = err
     out['split_speedup_vs_fp32'] = dict(estep=out['fp32']['estep_s'] / out['split']['estep_s'],
                                         gemm=out['fp32']['gemm_ms'] / out['split']['gemm_ms'])
+    out['split_stored_speedup_vs_fp32'] = dict(
+        estep=out['fp32']['estep_s'] / out['split_stored']['estep_s'],
+        gemm=out['fp32']['gemm_ms'] / out['split_stored']['gemm_ms'])
+    out['split']['ref_rows'] = ('generated in the GEMM kernel (k_gemm_split_gen); ref_rows_ms '
+                                'is the |R_d|^2 kernel')
     out['gemm'] = dict(M=R, N=N, K=2 * K, flops=useful,
                        note='cross term Re sum_p O_r(p) conj(R_d(p)) as a real GEMM over '
                             're/im-interleaved rows (K = 2 x scored pixels): 2 M N K flops; '
                             'split-BF16 executes 3 BF16 products per term (3x these flops on '
                             'MFMA)')
-    out['split']['mfma_bf16_tflops_executed'] = 3. * useful / (out['split']['gemm_ms'] / 1e3) / 1e12
+    for mode in ('split', 'split_stored'):
+        out[mode]['mfma_bf16_tflops_executed'] = 3. * useful / (out[mode]['gemm_ms'] / 1e3) / 1e12
     out['peaks_tflops'] = dict(bf16_dense=2500., fp32_vector=157.3, fp64=78.6)
     del plan, res
     torch.cuda.empty_cache()

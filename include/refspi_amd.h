@@ -254,7 +254,10 @@ int rsp_rot_set_frames(rsp_rot* plan, int64_t num_data, const int32_t* ones,
  * relative in Obar, for gradient refinement), 7 = precision of the model-gradient GEMM
  * GT = Q^T B of the adjoint (Q = prob * scale, B = the reference rows; 0 FP64 rocBLAS =
  * default, 1 split BF16 MFMA with FP32 accumulation over each sub-chunk of at most 12288
- * frames, sub-chunks added in FP64).  Refused while a pass is open. */
+ * frames, sub-chunks added in FP64), 8 = reference rows of the split cross term (1 =
+ * generated inside the GEMM kernel from sx, sy, dia and the pixel coordinates, never
+ * stored: the default for size <= 513; 0 = written to HBM as bf16 hi/lo rows and read by
+ * the GEMM; sizes above 513 always store them).  Refused while a pass is open. */
 int rsp_rot_set_option(rsp_rot* plan, int key, int64_t value);
 /* model_c: device c16 npix; sx, sy, dia, scales: device f64 N; ll, prob: device f64 N*R
  * (either may be NULL); logz: device f64 N (may be NULL).  Synchronises `stream`. */

@@ -59,6 +59,7 @@ typedef double f64x2v __attribute__((ext_vector_type(2)));
 constexpr double kPi = 3.14159265358979323846;
 
 enum Mode { kFP32 = 0, kSplit = 1, kFP64 = 2 };
+constexpr int kGenSizeMax = 513;  // generated reference rows: |x - c| <= 256 exact in bf16
 enum PassFlags { kPassMerge = 1, kPassFrameGrad = 2, kPassModelGrad = 4 };
 
 // Lane-iteration counts of the hypothesis kernels; the hypothesis stride is 64 * NT.
@@ -261,6 +262,30 @@ __global__ __launch_bounds__(256) void k_ref_rows(
     __syncthreads();
   }
   if (threadIdx.x == 0) normR[dl] = red[0];
+}
+
+// |R_d|^2 of frames [d0, d0 + gridDim.x) over the radius histogram: the sum over the
+// distinct radii of cnt * (rel S)^2 (the shift ramp has modulus 1), FP64, fixed reduction
+// order.  Used with the reference rows generated in the split GEMM (no rows are written).
+__global__ __launch_bounds__(256) void k_ref_norm(const double* __restrict__ rad,
+                                                  const int32_t* __restrict__ cnt, int nrad,
+                                                  double rel, int size,
+                                                  const double* __restrict__ dia, int64_t d0,
+                                                  double* __restrict__ normR) {
+  __shared__ double red[256];
+  const double fd = dia[d0 + blockIdx.x];
+  double acc = 0.;
+  for (int i = threadIdx.x; i < nrad; i += 256) {
+    const double sv = rel * sphere_guarded(rad[i] * kPi * fd / size);
+    acc += (double)cnt[i] * (sv * sv);
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) normR[blockIdx.x] = red[0];
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -872,13 +897,21 @@ struct rsp_rot {
   double *d_rcos = nullptr, *d_rsin = nullptr, *d_rad = nullptr;
   int32_t *d_scored = nullptr, *d_jrad = nullptr, *d_compact_of = nullptr;
   uint32_t* d_pxy = nullptr;  // scored pixel j as x | y << 16 (no division per pixel)
+  // split form with the reference rows generated in the GEMM (option 8): pixel j of the K
+  // order as bf16 (x - c) | bf16 (y - c) << 16 (Kp/2 entries), the sphere table, the
+  // number of scored pixels at each distinct radius, the largest squared radius
+  int gen_ref = 1;
+  uint32_t* d_gpix = nullptr;
+  float4* d_gtab = nullptr;
+  int32_t* d_radcnt = nullptr;
+  double r2max = 0.;
   // resident frames
   int64_t N = 0, nnz = 0;
   int64_t *d_fo = nullptr, *d_po = nullptr;
   int32_t *d_fj = nullptr, *d_pd = nullptr;
   float *d_fk = nullptr, *d_pk = nullptr;
   // forward work
-  int mode = kSplit, mode_alloc = -1;
+  int mode = kSplit, mode_alloc = -1, gen_alloc = -1;
   int64_t chunk = 0, chunk_alloc = 0;
   int xs = 1;                                   // K slices of the cross term (split form)
   void *d_A = nullptr, *d_OT = nullptr, *d_B = nullptr, *d_X = nullptr;
@@ -929,6 +962,7 @@ void free_work(rsp_rot* p) {
   dfree(p->d_part);
   dfree(p->d_normR);
   p->mode_alloc = -1;
+  p->gen_alloc = -1;
   p->chunk_alloc = 0;
 }
 
@@ -960,11 +994,16 @@ void free_merge(rsp_rot* p) {
 
 int64_t elem_bytes(int mode) { return mode == kFP64 ? 8 : 4; }
 
+// the B rows are stored (not generated in the GEMM)
+bool stores_b(const rsp_rot* p) { return !(p->mode == kSplit && p->gen_ref); }
+
 int ensure_work(rsp_rot* p, int64_t chunk) {
-  if (p->mode_alloc == p->mode && p->chunk_alloc == chunk) return kOk;
+  if (p->mode_alloc == p->mode && p->chunk_alloc == chunk && p->gen_alloc == p->gen_ref)
+    return kOk;
   free_work(p);
   const int64_t eb = elem_bytes(p->mode);
-  const int64_t Bpad = xterm_pad_n(chunk);
+  // no B buffer when the split GEMM generates its reference rows
+  const int64_t Bpad = stores_b(p) ? xterm_pad_n(chunk) : 1;
   const int64_t ntiles = ceil_div(p->K, 64);
   const int64_t ot_bytes = (int64_t)p->K * p->Hs * 16;
   const int xs = p->mode == kSplit ? xterm_split_k(p->Kp) : 1;
@@ -986,6 +1025,7 @@ int ensure_work(rsp_rot* p, int64_t chunk) {
   RSP_HIP(hipMemset(p->d_B, 0, (size_t)(Bpad * p->Kp * eb)));
   RSP_HIP(hipMemset(p->d_OT, 0, (size_t)ot_bytes));
   p->mode_alloc = p->mode;
+  p->gen_alloc = p->gen_ref;
   p->chunk_alloc = chunk;
   p->xs = xs;
   return kOk;
@@ -1120,12 +1160,17 @@ int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, cons
               hipStream_t st) {
   const int64_t chunk = p->chunk_alloc;
   const int64_t Bpad = xterm_pad_n(chunk);
+  const bool gen = !stores_b(p);
   int rc = kOk;
   if (p->timing && (rc = ensure_events(p))) return rc;
   for (int64_t d0 = 0; d0 < f.n; d0 += chunk) {
     const int64_t n = std::min(chunk, f.n - d0);
     if (p->timing) RSP_HIP(hipEventRecord(p->ev[2], st));
-    if (p->mode == kFP32)
+    if (gen) {
+      hipLaunchKernelGGL(k_ref_norm, dim3((unsigned)n), dim3(256), 0, st, p->d_rad,
+                         p->d_radcnt, p->nrad, p->rel, p->size, dia, d0, p->d_normR);
+      RSP_LAUNCH_CHECK("k_ref_norm");
+    } else if (p->mode == kFP32)
       rc = launch_ref_to<kFP32>(p, sx, sy, dia, d0, n, f.n, p->d_B, Bpad, p->d_normR, st);
     else if (p->mode == kSplit)
       rc = launch_ref_to<kSplit>(p, sx, sy, dia, d0, n, f.n, p->d_B, Bpad, p->d_normR, st);
@@ -1139,6 +1184,11 @@ int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, cons
     } else if (p->mode == kFP64) {
       rc = xterm_gemm_blas64(p->blas, (const double*)p->d_A, (const double*)p->d_B, (double*)p->d_X,
                              p->Mpad, n, p->Kp, p->Kp, p->Kp, p->Mpad, st);
+    } else if (gen) {
+      const __bf16* ah = (const __bf16*)p->d_A;
+      rc = xterm_gemm_split_gen(ah, ah + p->Mpad * p->Kp, p->d_gpix, p->d_gtab, p->r2max,
+                                p->rel, sx, sy, dia, d0, p->size, (float*)p->d_X, p->Mpad, n,
+                                p->Kp, p->Mpad, chunk * p->Mpad, st);
     } else {
       const __bf16* ah = (const __bf16*)p->d_A;
       const __bf16* bh = (const __bf16*)p->d_B;
@@ -1460,6 +1510,8 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
   p->R = (int)num_rot;
   set_hypotheses(p, 1);
   p->rel = rel_scale;
+  // the generated reference rows pack |x - c| <= 256 exactly in bf16: sizes up to 513
+  p->gen_ref = size <= kGenSizeMax ? 1 : 0;
   // scored pixels: probmask == 0 of emc.py:89-98
   const int c = (int)size / 2;
   p->compact_of.assign(p->npix, -1);
@@ -1494,14 +1546,39 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
     pxy[j] = (uint32_t)(p->scored[j] / size) | ((uint32_t)(p->scored[j] % size) << 16);
   p->nrad = (int)rad.size();
   p->Kp = xterm_pad_k(2 * (int64_t)p->K);
+  // generated reference rows: packed pixel coordinates of the K order (exact in bf16 for
+  // |x - c| <= 256), pixels per distinct radius, the largest squared radius
+  std::vector<uint32_t> gpix((size_t)(p->Kp / 2), 0u);
+  std::vector<int32_t> radcnt(p->nrad, 0);
+  auto bf16_bits = [](float v) {  // exact for the small integers stored here
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    return u >> 16;
+  };
+  for (int j = 0; j < p->K; ++j) {
+    const int xc = p->scored[j] / (int)size - c, yc = p->scored[j] % (int)size - c;
+    gpix[j] = bf16_bits((float)xc) | (bf16_bits((float)yc) << 16);
+    ++radcnt[jrad[j]];
+    p->r2max = std::max(p->r2max, (double)r2_of[j]);
+  }
   std::vector<double> rc(rot_cos, rot_cos + num_rot), rs(rot_sin, rot_sin + num_rot);
   int rc_ = kOk;
   if ((rc_ = dupload(&p->d_rcos, rc)) || (rc_ = dupload(&p->d_rsin, rs)) ||
       (rc_ = dupload(&p->d_rad, rad)) || (rc_ = dupload(&p->d_scored, p->scored)) ||
       (rc_ = dupload(&p->d_jrad, jrad)) || (rc_ = dupload(&p->d_compact_of, p->compact_of)) ||
-      (rc_ = dupload(&p->d_pxy, pxy))) {
+      (rc_ = dupload(&p->d_pxy, pxy)) || (rc_ = dupload(&p->d_gpix, gpix)) ||
+      (rc_ = dupload(&p->d_radcnt, radcnt)) ||
+      (rc_ = dalloc(&p->d_gtab, (int64_t)xterm_gen_table_len())) ||
+      (rc_ = xterm_sphere_table(p->rel, p->d_gtab, nullptr))) {
     rsp_rot_destroy(p);
     return rc_;
+  }
+  {
+    const hipError_t e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+      rsp_rot_destroy(p);
+      return hip_fail(e, "rsp_rot_create: sphere table");
+    }
   }
   if (sizeof(double) * (size_t)(p->nrad + 1) + 2 * sizeof(double2) * (size_t)size > 60000) {
     rsp_rot_destroy(p);
@@ -1593,6 +1670,12 @@ int rsp_rot_set_option(rsp_rot* p, int key, int64_t value) {
     case 6:
       if (value < 0 || value > 1) return fail(kErrSize, "frame-gradient GEMM must be 0 or 1");
       p->fg_split = (int)value;
+      return kOk;
+    case 8:
+      if (value < 0 || value > 1) return fail(kErrSize, "generated reference rows must be 0 or 1");
+      if (value && p->size > kGenSizeMax)
+        return fail(kErrUnsupported, "generated reference rows need size <= 513");
+      p->gen_ref = (int)value;  // the forward buffers follow at the next E-step
       return kOk;
     case 7:
       if (value < 0 || value > 1) return fail(kErrSize, "model-gradient GEMM must be 0 or 1");
@@ -1828,6 +1911,9 @@ int rsp_rot_destroy(rsp_rot* p) {
   dfree(p->d_rad);
   dfree(p->d_scored);
   dfree(p->d_pxy);
+  dfree(p->d_gpix);
+  dfree(p->d_gtab);
+  dfree(p->d_radcnt);
   dfree(p->d_jrad);
   dfree(p->d_compact_of);
   dfree(p->d_fo);

@@ -35,6 +35,9 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kTM = 256, kTN = 256, kBK = 32;
 constexpr int kRowB = kBK * 2;                 // bytes per tile row (64)
@@ -82,7 +85,8 @@ struct SplitGemm {
   int64_t aoff[kLA], boff[kLB];      // per-lane element offsets of this tile's slots
   int nk, wave, lane;
 
-  template <int S>
+  // WITH_B = false: only the A planes (the B tile is generated in LDS by the caller)
+  template <int S, bool WITH_B = true>
   __device__ __forceinline__ void issue(int q) const {
     const int64_t k0 = (int64_t)min(q, nk - 1) * kBK;
 #pragma unroll
@@ -90,10 +94,12 @@ struct SplitGemm {
       dma16(ah + aoff[u] + k0, t_ah<S>() + (wave * kLA + u) * 1024);
       dma16(al + aoff[u] + k0, t_al<S>() + (wave * kLA + u) * 1024);
     }
+    if constexpr (WITH_B) {
 #pragma unroll
-    for (int u = 0; u < kLB; ++u) {
-      dma16(bh + boff[u] + k0, t_bh<S>() + (wave * kLB + u) * 1024);
-      dma16(bl + boff[u] + k0, t_bl<S>() + (wave * kLB + u) * 1024);
+      for (int u = 0; u < kLB; ++u) {
+        dma16(bh + boff[u] + k0, t_bh<S>() + (wave * kLB + u) * 1024);
+        dma16(bl + boff[u] + k0, t_bl<S>() + (wave * kLB + u) * 1024);
+      }
     }
   }
 
@@ -121,6 +127,22 @@ struct SplitGemm {
   __device__ __forceinline__ static void mma(f32x16 (&acc)[4][4], const Frags& f) {
 #pragma unroll
     for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if constexpr (PR == 0)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+        else if constexpr (PR == 1)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+        else
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+      }
+  }
+
+  // Product PR on tile rows I0, I0 + 1 (8 MFMAs).
+  template <int PR, int I0>
+  __device__ __forceinline__ static void mma_rows(f32x16 (&acc)[4][4], const Frags& f) {
+#pragma unroll
+    for (int i = I0; i < I0 + 2; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         if constexpr (PR == 0)
@@ -247,6 +269,352 @@ __global__ __launch_bounds__(256, 1) void k_gemm_split(
   }
 }
 
+// ---- the split GEMM with its B operand generated in LDS -------------------------------
+// The rotation form's B rows are the reference terms R_d(j) = rel S(s) e^{2 pi i t},
+// s = pi r_j dia_d / size, t = ((x_j - c) sx_d + (y_j - c) sy_d) / size, which depend on three
+// numbers per frame and two per pixel.  k_gemm_split_gen computes each stage's B tile (256
+// frames x 16 pixels, bf16 hi/lo) in the registers of the thread that owns the frame's row
+// and writes it to the LDS buffers the A tile's loads do not use, in the layout the direct
+// loads would give, instead of reading 8 bytes per frame and pixel from HBM.
+//  * S is even in s, so rel S(s) = g(v), v = s^2 = r_j^2 (pi dia_d / size)^2 with r_j^2 an
+//    exact integer: g is tabulated once per plan (k_sphere_table, FP64) as kGTab cubic
+//    Hermite pieces on [0, kGVmax) (h = 0.5: interpolation error <= 1e-9, below the FP32
+//    evaluation's ~3e-8 near S = 1/3), and the table sits in LDS: one ds_read_b128 and
+//    three FMAs per element.  s < sqrt(kGVmax) = 22.6 covers diameters up to 14.4 px at the
+//    outermost scored pixel (r < size/2) of any grid; a wave holding a frame beyond it
+//    evaluates S directly (FP32 library sin/cos) where v leaves the table.
+//  * t in FP32 (|t| < 8 revolutions), reduced to [0, 1), cos and sin by v_cos_f32 /
+//    v_sin_f32 (their argument is in revolutions).
+//  * Pixels are packed one dword each: bf16 (x - c) | bf16 (y - c) << 16 (exact: |x - c| <= 256),
+//    read with scalar loads (uniform over the workgroup).  Padding pixels (j >= K) read 0:
+//    their A columns are zero, so their finite B values add nothing; rows of frames past the
+//    chunk give finite values in C rows that are never stored.
+constexpr int kGTab = 1024;
+constexpr double kGVmax = 512.;
+__shared__ __attribute__((aligned(16))) float4 s_gtab[kGTab];
+
+// DIRECT: the wave holds a frame whose pixels leave the table (a separate instantiation of
+// the stage loop, so that the table path stays straight-line).
+template <bool DIRECT>
+struct GenB {
+  const uint32_t* __restrict__ pix;  // this slice's pixels (16 per stage)
+  float a, b, kdx;                   // the row's frame: sx/size, sy/size, (pi dia/size)^2 / h
+  float kd2, rel;                    // (pi dia/size)^2 and rel, for the direct evaluation
+  int row, nk;
+
+  // The two elements of half hf of k-group kg (pixels 4kg + 2hf, +1: 8 bytes of each
+  // plane of the row) in two parts, so that the table reads of one half are in flight
+  // while the MFMAs of a region run: prep issues them, finish forms the elements.
+  struct Prep {
+    float f[2], x[2], r2[2];
+    float4 c[2];
+  };
+
+  __device__ __forceinline__ static void coords(uint32_t w, float& xc, float& yc) {
+    xc = __uint_as_float(w << 16);
+    yc = __uint_as_float(w & 0xffff0000u);
+  }
+
+  __device__ __forceinline__ void prep(const uint32_t (&w)[16], int kg, int hf, Prep& p) const {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      float xc, yc;
+      coords(w[4 * kg + 2 * hf + u], xc, yc);
+      const float r2 = fmaf(xc, xc, yc * yc);  // exact (integers below 2^24)
+      const float x = r2 * kdx;                // table coordinate v / h
+      const int i = min((int)x, kGTab - 1);    // x >= 0: truncation is the floor
+      p.f[u] = x - (float)i;
+      p.c[u] = s_gtab[i];
+      if constexpr (DIRECT) {
+        p.x[u] = x;
+        p.r2[u] = r2;
+      }
+    }
+  }
+
+  // g(v): the cubic piece, or (DIRECT, past the table) rel S(s) = rel (sin s - s cos s) / s^3
+  // with s = sqrt(v) > 22.6 (no cancellation), by the hardware sine and cosine (argument in
+  // revolutions)
+  template <int S>
+  __device__ __forceinline__ void finish(const uint32_t (&w)[16], int kg, int hf,
+                                         const Prep& p) const {
+    u32x2 hv, lv;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      float xc, yc;
+      coords(w[4 * kg + 2 * hf + u], xc, yc);
+      const float4 c = p.c[u];
+      const float f = p.f[u];
+      float g = fmaf(fmaf(fmaf(c.w, f, c.z), f, c.y), f, c.x);
+      if constexpr (DIRECT) {
+        const float sv = sqrtf(p.r2[u] * kd2);
+        const float t = sv * 0.15915494309189535f;  // s / (2 pi)
+        const float gd = rel * (__builtin_amdgcn_sinf(t) - sv * __builtin_amdgcn_cosf(t)) /
+                         (sv * sv * sv);
+        g = p.x[u] > (float)kGTab ? gd : g;
+      }
+      const float t = __builtin_amdgcn_fractf(fmaf(xc, a, yc * b));
+      const f32x2 v = {g * __builtin_amdgcn_cosf(t), g * __builtin_amdgcn_sinf(t)};
+      // bf16 hi = (re, im) rounded, lo = the rounded remainders (one conversion per pair)
+      const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+      const f32x2 r = {v[0] - __uint_as_float(h << 16), v[1] - __uint_as_float(h & 0xffff0000u)};
+      hv[u] = h;
+      lv[u] = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+    }
+    // the row's swizzle as an opaque value: the 16 write offsets (2 buffers x 8 halves) are
+    // then formed at each write instead of held in registers across the loop
+    int sw = swz(row);
+    asm volatile("" : "+v"(sw));
+    const int o = row * kRowB + ((kg ^ sw) << 4) + 8 * hf;
+    *reinterpret_cast<u32x2*>(t_bh<S>() + o) = hv;
+    *reinterpret_cast<u32x2*>(t_bl<S>() + o) = lv;
+  }
+
+  // The 16 pixel words of stage q (uniform: scalar loads, read once per stage).  Stages
+  // past the slice (q >= nk) recompute the last one into a buffer no stage reads.
+  __device__ __forceinline__ void words(int q, uint32_t (&w)[16]) const {
+    const uint32_t* pw = pix + (int64_t)min(q, nk - 1) * (kBK / 2);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) w[u] = pw[u];
+  }
+
+  // k-group kg (pixels 4kg .. 4kg+3 of a stage: 8 k values, re/im interleaved) of this
+  // thread's row goes to slot kg ^ swz(row) of the row's 64 bytes in the stage's B buffers.
+  template <int S>
+  __device__ __forceinline__ void half(const uint32_t (&w)[16], int kg, int hf) const {
+    Prep p;
+    prep(w, kg, hf, p);
+    finish<S>(w, kg, hf, p);
+  }
+};
+
+// The A tile's direct-to-LDS loads with a wave-uniform base and one 32-bit byte offset per
+// lane and slot (no 64-bit address registers).
+struct AStream {
+  const char *ah, *al;   // this tile's first row at the slice's first k
+  uint32_t off[kLA];     // byte offsets of this lane's 16-byte slots within the tile
+  int nk, wave;
+
+  template <int S>
+  __device__ __forceinline__ void issue(int q) const {
+    const int64_t kb = (int64_t)min(q, nk - 1) * kBK * 2;
+#pragma unroll
+    for (int u = 0; u < kLA; ++u) {
+      dma16(ah + kb + off[u], t_ah<S>() + (wave * kLA + u) * 1024);
+      dma16(al + kb + off[u], t_al<S>() + (wave * kLA + u) * 1024);
+    }
+  }
+};
+
+// One stage of the wave's tile (the products of SplitGemm::compute) with the generation of
+// the next stage's B tile (buffer SB).  The generation runs only while one fragment set is
+// live: regions of 8 MFMAs with two generated elements each over the first two products of
+// the first 16-k step and of the second; the second step's fragment reads are issued during
+// the first step's third product, which carries no generation.
+template <int S, int SB, bool DIRECT>
+__device__ __forceinline__ void compute_gen(const SplitGemm& g, const GenB<DIRECT>& gb, int qn,
+                                            f32x16 (&acc)[4][4], int wm, int wn) {
+  uint32_t w[16];
+  gb.words(qn, w);
+  Frags f0, f1;
+  g.load<S>(f0, 0, wm, wn);
+  asm volatile(""
+               : "+v"(f0.ah[0]), "+v"(f0.ah[1]), "+v"(f0.ah[2]), "+v"(f0.ah[3]),
+                 "+v"(f0.al[0]), "+v"(f0.al[1]), "+v"(f0.al[2]), "+v"(f0.al[3]),
+                 "+v"(f0.bh[0]), "+v"(f0.bh[1]), "+v"(f0.bh[2]), "+v"(f0.bh[3]),
+                 "+v"(f0.bl[0]), "+v"(f0.bl[1]), "+v"(f0.bl[2]), "+v"(f0.bl[3])
+               :
+               : "memory");
+  typename GenB<DIRECT>::Prep pa, pb;
+  gb.prep(w, 0, 0, pa);
+  // a region: 8 MFMAs, the table reads of the next half and the VALU work of this one
+  // between them
+#define RSP_GEN_REGION(PR, I0, F, KG, HF, PN, KGN, HFN, PC)                  \
+  SplitGemm::mma_rows<PR, I0>(acc, F);                                      \
+  if (KGN >= 0) gb.prep(w, KGN, HFN, PN);                                   \
+  gb.template finish<SB>(w, KG, HF, PC);                                    \
+  for (int u_ = 0; u_ < 8; ++u_) {                                          \
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* one MFMA */       \
+    __builtin_amdgcn_sched_group_barrier(0x002, 5, 0); /* five VALU */      \
+  }                                                                         \
+  __builtin_amdgcn_sched_barrier(0);
+  RSP_GEN_REGION(0, 0, f0, 0, 0, pb, 0, 1, pa)
+  RSP_GEN_REGION(0, 2, f0, 0, 1, pa, 1, 0, pb)
+  RSP_GEN_REGION(1, 0, f0, 1, 0, pb, 1, 1, pa)
+  RSP_GEN_REGION(1, 2, f0, 1, 1, pa, 2, 0, pb)
+  g.load<S>(f1, 1, wm, wn);
+  SplitGemm::mma<2>(acc, f0);
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  RSP_GEN_REGION(0, 0, f1, 2, 0, pb, 2, 1, pa)
+  RSP_GEN_REGION(0, 2, f1, 2, 1, pa, 3, 0, pb)
+  RSP_GEN_REGION(1, 0, f1, 3, 0, pb, 3, 1, pa)
+  RSP_GEN_REGION(1, 2, f1, 3, 1, pa, -1, 0, pb)
+#undef RSP_GEN_REGION
+  SplitGemm::mma<2>(acc, f1);
+}
+
+// The stage loop of k_gemm_split_gen (accumulators zeroed here).
+template <bool DIRECT>
+__device__ __forceinline__ void gen_loop(const SplitGemm& g, const AStream& as,
+                                         const GenB<DIRECT>& gb, f32x16 (&acc)[4][4], int wm,
+                                         int wn) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
+  as.issue<0>(0);
+  {
+    uint32_t w[16];
+    gb.words(0, w);
+#pragma unroll
+    for (int kg = 0; kg < 4; ++kg) {
+      gb.template half<0>(w, kg, 0);
+      gb.template half<0>(w, kg, 1);
+    }
+  }
+  for (int q = 0; q < g.nk; q += 2) {
+    // vmcnt = 0: this wave's A loads of stage q have landed; lgkmcnt = 0: its B-tile writes
+    // of stage q and its fragment reads of stage q-1 are done (the buffers of stage q+1
+    // are rewritten after the barrier)
+    __builtin_amdgcn_s_waitcnt(0x070);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    as.issue<1>(q + 1);
+    compute_gen<0, 1>(g, gb, q + 1, acc, wm, wn);
+    __builtin_amdgcn_s_waitcnt(0x070);
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    as.issue<0>(q + 2);
+    if (q + 1 < g.nk) compute_gen<1, 0>(g, gb, q + 2, acc, wm, wn);
+  }
+}
+
+// X partial products of the K slices (as k_gemm_split) with B = the reference rows of frames
+// d0 .. d0 + N - 1 (sx, sy, dia indexed by frame), generated per stage.  gtab: kGTab cubic
+// pieces of g and, at gtab[kGTab], {1 / h, v_max, 0, 0} (k_sphere_table); r2max: the largest
+// squared radius of the scored pixels.
+__global__ __launch_bounds__(256, 1) void k_gemm_split_gen(
+    const __bf16* __restrict__ ah, const __bf16* __restrict__ al,
+    const uint32_t* __restrict__ pix, const float4* __restrict__ gtab, float r2max, float rel,
+    const double* __restrict__ sx, const double* __restrict__ sy,
+    const double* __restrict__ dia, int64_t d0, int size, float* __restrict__ C, int64_t N,
+    int64_t Kp, int64_t ldc, int nM, int nN, int kst, int64_t slice_stride) {
+  const int T = nM * nN;
+  const int T8 = (T + 7) / 8;
+  const int task = (int)(blockIdx.x & 7) * T8 + (int)(blockIdx.x >> 3);
+  if (task >= T) return;
+  const int mt = task % nM, nt = task / nM;
+  const int64_t m0 = (int64_t)mt * kTM, n0 = (int64_t)nt * kTN;
+  const int z = blockIdx.y;
+  const int nkt = (int)(Kp / kBK);
+  const int64_t kz = (int64_t)z * kst * kBK;
+  C += (int64_t)z * slice_stride;
+  SplitGemm g;   // fragment reads and products (its load addresses are unused)
+  g.lane = threadIdx.x & 63;
+  g.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  g.nk = min(kst, nkt - z * kst);
+  AStream as;
+  as.ah = reinterpret_cast<const char*>(ah + m0 * Kp + kz);
+  as.al = reinterpret_cast<const char*>(al + m0 * Kp + kz);
+  as.nk = g.nk;
+  as.wave = g.wave;
+#pragma unroll
+  for (int u = 0; u < kLA; ++u) {
+    const int s = (g.wave * kLA + u) * 64 + g.lane;
+    const int row = s >> 2, kcl = (s & 3) ^ swz(row);
+    as.off[u] = (uint32_t)(((int64_t)row * Kp + kcl * 8) * 2);
+  }
+  // the sphere table into LDS
+  for (int i = threadIdx.x; i < kGTab; i += 256) s_gtab[i] = gtab[i];
+  const float inv_h = gtab[kGTab].x;
+  GenB<false> gb;
+  gb.pix = pix + kz / 2;
+  gb.row = threadIdx.x;
+  gb.nk = g.nk;
+  {
+    const int64_t r = n0 + threadIdx.x;  // the row's frame within the chunk
+    double fa = 0., fb = 0., fk = 0.;
+    if (r < N) {
+      const double kd = 3.14159265358979323846 * dia[d0 + r] / size;
+      fa = sx[d0 + r] / size;
+      fb = sy[d0 + r] / size;
+      fk = kd * kd * inv_h;
+    }
+    gb.a = (float)fa;
+    gb.b = (float)fb;
+    gb.kdx = (float)fk;
+    gb.kd2 = (float)(fk / inv_h);
+    gb.rel = rel;
+  }
+  __syncthreads();
+  const int wm = g.wave >> 1, wn = g.wave & 1;
+  f32x16 acc[4][4];
+  // a frame whose outermost pixel leaves the table sends its wave to the direct path
+  if (__any(r2max * gb.kdx > (float)kGTab)) {
+    const GenB<true> gd{gb.pix, gb.a, gb.b, gb.kdx, gb.kd2, gb.rel, gb.row, gb.nk};
+    gen_loop(g, as, gd, acc, wm, wn);
+  } else {
+    gen_loop(g, as, gb, acc, wm, wn);
+  }
+  __builtin_amdgcn_s_waitcnt(0xF70);   // the trailing (clamped) loads
+  const int lr = g.lane & 31, hi = g.lane >> 5;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t d = n0 + wn * 128 + j * 32 + lr;
+    if (d >= N) continue;
+    float* crow = C + d * ldc + m0 + wm * 128;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int e4 = 0; e4 < 4; ++e4) {
+        f32x4 v = {acc[i][j][4 * e4], acc[i][j][4 * e4 + 1], acc[i][j][4 * e4 + 2],
+                   acc[i][j][4 * e4 + 3]};
+        *reinterpret_cast<f32x4*>(crow + i * 32 + 8 * e4 + 4 * hi) = v;
+      }
+  }
+}
+
+// rel S(s) and dS/ds in FP64 (the s == 0 limit 1/3 by its series below s = 0.5)
+__device__ __forceinline__ void sphere_g(double v, double& g, double& dg) {
+  const double s = sqrt(v);
+  if (s < 0.5) {
+    // S = 1/3 - v/30 + v^2/840 - v^3/45360 + v^4/3991680 - v^5/518918400
+    g = 1. / 3. + v * (-1. / 30. + v * (1. / 840. + v * (-1. / 45360. +
+                 v * (1. / 3991680. - v / 518918400.))));
+    dg = -1. / 30. + v * (2. / 840. + v * (-3. / 45360. + v * (4. / 3991680. -
+         5. * v / 518918400.)));
+  } else {
+    double sn, cs;
+    sincos(s, &sn, &cs);
+    g = (sn - s * cs) / (s * s * s);
+    // dS/ds = sin s / s^2 - 3 S / s;  dg/dv = dS/ds / (2 s)
+    dg = (sn / (s * s) - 3. * g / s) / (2. * s);
+  }
+}
+
+// The table of g(v) = rel S(sqrt v) for k_gemm_split_gen: h = kGVmax / kGTab, piece i on
+// [i h, (i+1) h) as c0 + c1 f + c2 f^2 + c3 f^3, f = v/h - i (cubic Hermite from g and dg/dv
+// at both ends, FP64, rounded to FP32); out[kGTab] = {1/h, kGVmax, 0, 0}.
+__global__ __launch_bounds__(256) void k_sphere_table(double rel, float4* __restrict__ out) {
+  const double h = kGVmax / kGTab;
+  for (int i = threadIdx.x; i < kGTab; i += 256) {
+    double g0, d0, g1, d1;
+    sphere_g(i * h, g0, d0);
+    sphere_g((i + 1) * h, g1, d1);
+    g0 *= rel; g1 *= rel; d0 *= rel * h; d1 *= rel * h;
+    out[i] = make_float4((float)g0, (float)d0, (float)(3. * (g1 - g0) - 2. * d0 - d1),
+                         (float)(2. * (g0 - g1) + d0 + d1));
+  }
+  if (threadIdx.x == 0) out[kGTab] = make_float4((float)(1. / h), (float)kGVmax, 0.f, 0.f);
+}
+
 // FP32 rows [rows x K] (leading dimension ld) -> bf16 hi/lo planes [rows_pad x Kp],
 // zero padded.  One thread per 8 consecutive k.
 __global__ void k_split_rows(const float* __restrict__ X, int64_t rows, int64_t K, int64_t ld,
@@ -343,6 +711,34 @@ int xterm_gemm_split(const __bf16* ah, const __bf16* al, const __bf16* bh, const
   hipLaunchKernelGGL(k_gemm_split, dim3(8 * T8, S), dim3(256), 0, stream, ah, al, bh, bl, C, N,
                      Kp, ldc, nM, nN, kSliceStages, slice_stride);
   RSP_LAUNCH_CHECK("k_gemm_split");
+  return kOk;
+}
+
+int xterm_gen_table_len() { return kGTab + 1; }
+
+int xterm_sphere_table(double rel, float4* table, hipStream_t stream) {
+  hipLaunchKernelGGL(k_sphere_table, dim3(1), dim3(256), 0, stream, rel, table);
+  RSP_LAUNCH_CHECK("k_sphere_table");
+  return kOk;
+}
+
+// Split-BF16 GEMM whose B rows are the reference terms of frames d0 .. d0 + N - 1, generated
+// in the kernel (pixels packed by the caller, table from xterm_sphere_table).
+int xterm_gemm_split_gen(const __bf16* ah, const __bf16* al, const uint32_t* pix,
+                         const float4* table, double r2max, double rel, const double* sx,
+                         const double* sy, const double* dia, int64_t d0, int size, float* C,
+                         int64_t Mpad, int64_t N, int64_t Kp, int64_t ldc, int64_t slice_stride,
+                         hipStream_t stream) {
+  if (Mpad % kTM || Kp % kBK || ldc < Mpad || ldc % 4) return fail(kErrSize, "gemm padding");
+  const int S = xterm_split_k(Kp);
+  if (S > 1 && slice_stride < (N - 1) * ldc + Mpad) return fail(kErrSize, "gemm slice stride");
+  if (N == 0 || Mpad == 0) return kOk;
+  const int nM = (int)(Mpad / kTM), nN = (int)ceil_div(N, kTN);
+  const int T8 = (nM * nN + 7) / 8;
+  hipLaunchKernelGGL(k_gemm_split_gen, dim3(8 * T8, S), dim3(256), 0, stream, ah, al, pix,
+                     table, (float)r2max, (float)rel, sx, sy, dia, d0, size, C, N, Kp, ldc, nM,
+                     nN, kSliceStages, slice_stride);
+  RSP_LAUNCH_CHECK("k_gemm_split_gen");
   return kOk;
 }
 

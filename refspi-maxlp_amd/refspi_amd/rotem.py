@@ -83,6 +83,15 @@ class _Plan:
         _lib.call('rsp_rot_set_option', self._h, 6, 1 if precision == 'split' else 0)
         self.frame_grad = precision
 
+    def set_ref_rows(self, how):
+        """Reference rows R_d of the split cross term: 'generated' (computed inside the GEMM
+        kernel from three numbers per frame, never stored; the default up to 513 x 513) or
+        'stored' (written to HBM as bf16 hi/lo rows, then read by the GEMM)."""
+        if how not in ('generated', 'stored'):
+            raise ValueError("reference rows must be 'generated' or 'stored'")
+        _lib.call('rsp_rot_set_option', self._h, 8, 1 if how == 'generated' else 0)
+        self.ref_rows = how
+
     def set_model_grad(self, precision):
         """Precision of the model-gradient GEMM GT = Q^T B of the adjoint (Q = P * scale,
         B = the reference rows): 'fp64' (rocBLAS, the default) or 'split' (BF16 hi/lo on

@@ -45,10 +45,14 @@ def _plan(frames, mode, **kw):
     return RotEM(Dataset(frames=frames), SMALL['size'], R, rel_scale=1.3, mode=mode, **kw)
 
 
-@pytest.mark.parametrize('mode', ['fp64', 'fp32', 'split'])
+@pytest.mark.parametrize('mode', ['fp64', 'fp32', 'split', 'split-stored'])
 def test_rot_estep_matches_oracle(small_frames, oracle_small, mode):
+    """split: the reference rows generated in the GEMM kernel (the default); split-stored:
+    written to HBM and read by the GEMM."""
     o = oracle_small
-    plan = _plan(small_frames, mode)
+    plan = _plan(small_frames, mode.split('-')[0])
+    if mode == 'split-stored':
+        plan.set_ref_rows('stored')
     ll, prob, logz = plan.estep(o['model'], o['sx'], o['sy'], o['dia'], o['sc'])
     err = np.abs(ll - o['LL']) / np.abs(o['LL'])
     if mode == 'fp64':
@@ -102,12 +106,39 @@ def test_rot_split_vs_fp64_at_185(tmp_path):
     model = (rng.standard_normal(size * size) + 1j * rng.standard_normal(size * size)) * 0.3
     ds = Dataset(frames=fr)
     res = {}
-    for mode in ('fp64', 'fp32', 'split'):
-        plan = RotEM(ds, size, 360, rel_scale=1., mode=mode)
+    for mode in ('fp64', 'fp32', 'split', 'split-stored'):
+        plan = RotEM(ds, size, 360, rel_scale=1., mode=mode.split('-')[0])
+        if mode == 'split-stored':
+            plan.set_ref_rows('stored')
         res[mode] = plan.estep(model, t['shifts'][:, 0], t['shifts'][:, 1], t['diameters'])
         del plan
         torch.cuda.synchronize()
     ll64 = res['fp64'][0]
-    for mode in ('fp32', 'split'):
+    errs = {}
+    for mode in ('fp32', 'split', 'split-stored'):
         err = np.abs(res[mode][0] - ll64) / np.abs(ll64)
+        errs[mode] = err.max()
         assert err.max() < 1e-5, (mode, err.max())
+    print('\nLL relative error against FP64:', errs)
+
+
+@pytest.mark.parametrize('dias', ['wide', 'zero'])
+def test_rot_generated_ref_rows_edge_diameters(small_frames, oracle_small, dias):
+    """Generated reference rows over unusual diameters: a wide spread (the sphere table
+    spans the largest diameter's range, 25 px here) and frames of diameter 0 (S = 1/3 at
+    s = 0, the reference's s == 0 guard): LL within 1e-5 of the FP64 oracle."""
+    from oracle import rot_oracle as O
+    o = oracle_small
+    dia = o['dia'].copy()
+    if dias == 'wide':
+        dia[::3] = 25.
+        dia[1::7] = 0.5
+    else:
+        dia[::4] = 0.
+    size = SMALL['size']
+    LL, _ = O.rot_loglik(o['model'], size, R, o['K'], o['sx'], o['sy'], dia, o['sc'],
+                         rel_scale=1.3)
+    plan = _plan(small_frames, 'split')
+    ll, _, _ = plan.estep(o['model'], o['sx'], o['sy'], dia, o['sc'])
+    err = np.abs(ll - LL) / np.abs(LL)
+    assert err.max() < 1e-5, err.max()
