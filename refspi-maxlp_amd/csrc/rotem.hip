@@ -276,7 +276,12 @@ __global__ __launch_bounds__(256) void k_ref_norm(const double* __restrict__ rad
   const double fd = dia[d0 + blockIdx.x];
   double acc = 0.;
   for (int i = threadIdx.x; i < nrad; i += 256) {
-    const double sv = rel * sphere_guarded(rad[i] * kPi * fd / size);
+    // sphere_guarded with one sincos
+    double s = rad[i] * kPi * fd / size;
+    if (s == 0.) s = 1.e-5;
+    double sn, cs;
+    sincos(s, &sn, &cs);
+    const double sv = rel * ((sn - s * cs) / (s * s * s));
     acc += (double)cnt[i] * (sv * sv);
   }
   red[threadIdx.x] = acc;
@@ -898,10 +903,12 @@ struct rsp_rot {
   int32_t *d_scored = nullptr, *d_jrad = nullptr, *d_compact_of = nullptr;
   uint32_t* d_pxy = nullptr;  // scored pixel j as x | y << 16 (no division per pixel)
   // split form with the reference rows generated in the GEMM (option 8): pixel j of the K
-  // order as bf16 (x - c) | bf16 (y - c) << 16 (Kp/2 entries), the sphere table, the
-  // number of scored pixels at each distinct radius, the largest squared radius
+  // order as bf16 (x - c) | bf16 (y - c) << 16 and its squared radius (Kp/2 entries), the
+  // sphere table, the number of scored pixels at each distinct radius, the largest squared
+  // radius
   int gen_ref = 1;
   uint32_t* d_gpix = nullptr;
+  float* d_gr2 = nullptr;
   float4* d_gtab = nullptr;
   int32_t* d_radcnt = nullptr;
   double r2max = 0.;
@@ -1186,7 +1193,7 @@ int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, cons
                              p->Mpad, n, p->Kp, p->Kp, p->Kp, p->Mpad, st);
     } else if (gen) {
       const __bf16* ah = (const __bf16*)p->d_A;
-      rc = xterm_gemm_split_gen(ah, ah + p->Mpad * p->Kp, p->d_gpix, p->d_gtab, p->r2max,
+      rc = xterm_gemm_split_gen(ah, ah + p->Mpad * p->Kp, p->d_gpix, p->d_gr2, p->d_gtab, p->r2max,
                                 p->rel, sx, sy, dia, d0, p->size, (float*)p->d_X, p->Mpad, n,
                                 p->Kp, p->Mpad, chunk * p->Mpad, st);
     } else {
@@ -1404,9 +1411,14 @@ int run_adj_finish(rsp_rot* p, double* g_model, hipStream_t st) {
 int64_t default_chunk(const rsp_rot* p, int64_t nframes) {
   int64_t chunk = p->chunk;
   if (chunk <= 0) {
-    // operand rows of a chunk within about 8 GB
-    chunk = std::max<int64_t>(128, (int64_t)(8e9 / (double)(p->Kp * elem_bytes(p->mode))) /
-                                       128 * 128);
+    // operand rows of a chunk within about 8 GB; with the reference rows generated in the
+    // GEMM, its K-slice partial products and log-likelihoods within about 8 GB (one chunk
+    // for 200k frames at 256^2: fewer, larger launches; the last wave of work-groups is a
+    // smaller share of each)
+    const double per_frame =
+        stores_b(p) ? (double)(p->Kp * elem_bytes(p->mode))
+                    : (double)(xterm_split_k(p->Kp) * p->Mpad * 4 + (int64_t)p->H * 8);
+    chunk = std::max<int64_t>(128, (int64_t)(8e9 / per_frame) / 128 * 128);
   }
   return std::min(chunk, xterm_pad_n(nframes));
 }
@@ -1549,6 +1561,7 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
   // generated reference rows: packed pixel coordinates of the K order (exact in bf16 for
   // |x - c| <= 256), pixels per distinct radius, the largest squared radius
   std::vector<uint32_t> gpix((size_t)(p->Kp / 2), 0u);
+  std::vector<float> gr2((size_t)(p->Kp / 2), 0.f);
   std::vector<int32_t> radcnt(p->nrad, 0);
   auto bf16_bits = [](float v) {  // exact for the small integers stored here
     uint32_t u;
@@ -1558,6 +1571,7 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
   for (int j = 0; j < p->K; ++j) {
     const int xc = p->scored[j] / (int)size - c, yc = p->scored[j] % (int)size - c;
     gpix[j] = bf16_bits((float)xc) | (bf16_bits((float)yc) << 16);
+    gr2[j] = (float)r2_of[j];  // exact: below 2^24
     ++radcnt[jrad[j]];
     p->r2max = std::max(p->r2max, (double)r2_of[j]);
   }
@@ -1567,6 +1581,7 @@ int rsp_rot_create(int device, int64_t size, int64_t num_rot, const double* rot_
       (rc_ = dupload(&p->d_rad, rad)) || (rc_ = dupload(&p->d_scored, p->scored)) ||
       (rc_ = dupload(&p->d_jrad, jrad)) || (rc_ = dupload(&p->d_compact_of, p->compact_of)) ||
       (rc_ = dupload(&p->d_pxy, pxy)) || (rc_ = dupload(&p->d_gpix, gpix)) ||
+      (rc_ = dupload(&p->d_gr2, gr2)) ||
       (rc_ = dupload(&p->d_radcnt, radcnt)) ||
       (rc_ = dalloc(&p->d_gtab, (int64_t)xterm_gen_table_len())) ||
       (rc_ = xterm_sphere_table(p->rel, p->d_gtab, nullptr))) {
@@ -1912,6 +1927,7 @@ int rsp_rot_destroy(rsp_rot* p) {
   dfree(p->d_scored);
   dfree(p->d_pxy);
   dfree(p->d_gpix);
+  dfree(p->d_gr2);
   dfree(p->d_gtab);
   dfree(p->d_radcnt);
   dfree(p->d_jrad);

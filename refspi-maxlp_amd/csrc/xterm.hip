@@ -298,6 +298,7 @@ __shared__ __attribute__((aligned(16))) float4 s_gtab[kGTab];
 template <bool DIRECT>
 struct GenB {
   const uint32_t* __restrict__ pix;  // this slice's pixels (16 per stage)
+  const float* __restrict__ pr2;     // their squared radii (exact integers)
   float a, b, kdx;                   // the row's frame: sx/size, sy/size, (pi dia/size)^2 / h
   float kd2, rel;                    // (pi dia/size)^2 and rel, for the direct evaluation
   int row, nk;
@@ -315,19 +316,17 @@ struct GenB {
     yc = __uint_as_float(w & 0xffff0000u);
   }
 
-  __device__ __forceinline__ void prep(const uint32_t (&w)[16], int kg, int hf, Prep& p) const {
+  __device__ __forceinline__ void prep(const float (&r2)[16], int kg, int hf, Prep& p) const {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      float xc, yc;
-      coords(w[4 * kg + 2 * hf + u], xc, yc);
-      const float r2 = fmaf(xc, xc, yc * yc);  // exact (integers below 2^24)
-      const float x = r2 * kdx;                // table coordinate v / h
-      const int i = min((int)x, kGTab - 1);    // x >= 0: truncation is the floor
-      p.f[u] = x - (float)i;
+      const float x = r2[4 * kg + 2 * hf + u] * kdx;  // table coordinate v / h (>= 0)
+      // x < kGTab - 1/2 outside DIRECT waves: the piece is x's integer part, f its fraction
+      const int i = DIRECT ? min((int)x, kGTab - 1) : (int)x;
+      p.f[u] = DIRECT ? x - (float)i : __builtin_amdgcn_fractf(x);
       p.c[u] = s_gtab[i];
       if constexpr (DIRECT) {
         p.x[u] = x;
-        p.r2[u] = r2;
+        p.r2[u] = r2[4 * kg + 2 * hf + u];
       }
     }
   }
@@ -370,20 +369,25 @@ struct GenB {
     *reinterpret_cast<u32x2*>(t_bl<S>() + o) = lv;
   }
 
-  // The 16 pixel words of stage q (uniform: scalar loads, read once per stage).  Stages
-  // past the slice (q >= nk) recompute the last one into a buffer no stage reads.
-  __device__ __forceinline__ void words(int q, uint32_t (&w)[16]) const {
-    const uint32_t* pw = pix + (int64_t)min(q, nk - 1) * (kBK / 2);
+  // The 16 pixel words and squared radii of stage q (uniform: scalar loads, read once per
+  // stage).  Stages past the slice (q >= nk) recompute the last one into a buffer no stage
+  // reads.
+  __device__ __forceinline__ void words(int q, uint32_t (&w)[16], float (&r2)[16]) const {
+    const int64_t o = (int64_t)min(q, nk - 1) * (kBK / 2);
 #pragma unroll
-    for (int u = 0; u < 16; ++u) w[u] = pw[u];
+    for (int u = 0; u < 16; ++u) {
+      w[u] = pix[o + u];
+      r2[u] = pr2[o + u];
+    }
   }
 
   // k-group kg (pixels 4kg .. 4kg+3 of a stage: 8 k values, re/im interleaved) of this
   // thread's row goes to slot kg ^ swz(row) of the row's 64 bytes in the stage's B buffers.
   template <int S>
-  __device__ __forceinline__ void half(const uint32_t (&w)[16], int kg, int hf) const {
+  __device__ __forceinline__ void half(const uint32_t (&w)[16], const float (&r2)[16], int kg,
+                                       int hf) const {
     Prep p;
-    prep(w, kg, hf, p);
+    prep(r2, kg, hf, p);
     finish<S>(w, kg, hf, p);
   }
 };
@@ -415,7 +419,8 @@ template <int S, int SB, bool DIRECT>
 __device__ __forceinline__ void compute_gen(const SplitGemm& g, const GenB<DIRECT>& gb, int qn,
                                             f32x16 (&acc)[4][4], int wm, int wn) {
   uint32_t w[16];
-  gb.words(qn, w);
+  float r2[16];
+  gb.words(qn, w, r2);
   Frags f0, f1;
   g.load<S>(f0, 0, wm, wn);
   asm volatile(""
@@ -426,12 +431,12 @@ __device__ __forceinline__ void compute_gen(const SplitGemm& g, const GenB<DIREC
                :
                : "memory");
   typename GenB<DIRECT>::Prep pa, pb;
-  gb.prep(w, 0, 0, pa);
+  gb.prep(r2, 0, 0, pa);
   // a region: 8 MFMAs, the table reads of the next half and the VALU work of this one
   // between them
 #define RSP_GEN_REGION(PR, I0, F, KG, HF, PN, KGN, HFN, PC)                  \
   SplitGemm::mma_rows<PR, I0>(acc, F);                                      \
-  if (KGN >= 0) gb.prep(w, KGN, HFN, PN);                                   \
+  if (KGN >= 0) gb.prep(r2, KGN, HFN, PN);                                  \
   gb.template finish<SB>(w, KG, HF, PC);                                    \
   for (int u_ = 0; u_ < 8; ++u_) {                                          \
     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* one MFMA */       \
@@ -472,11 +477,12 @@ __device__ __forceinline__ void gen_loop(const SplitGemm& g, const AStream& as,
   as.issue<0>(0);
   {
     uint32_t w[16];
-    gb.words(0, w);
+    float r2[16];
+    gb.words(0, w, r2);
 #pragma unroll
     for (int kg = 0; kg < 4; ++kg) {
-      gb.template half<0>(w, kg, 0);
-      gb.template half<0>(w, kg, 1);
+      gb.template half<0>(w, r2, kg, 0);
+      gb.template half<0>(w, r2, kg, 1);
     }
   }
   for (int q = 0; q < g.nk; q += 2) {
@@ -502,7 +508,8 @@ __device__ __forceinline__ void gen_loop(const SplitGemm& g, const AStream& as,
 // squared radius of the scored pixels.
 __global__ __launch_bounds__(256, 1) void k_gemm_split_gen(
     const __bf16* __restrict__ ah, const __bf16* __restrict__ al,
-    const uint32_t* __restrict__ pix, const float4* __restrict__ gtab, float r2max, float rel,
+    const uint32_t* __restrict__ pix, const float* __restrict__ pr2,
+    const float4* __restrict__ gtab, float r2max, float rel,
     const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ dia, int64_t d0, int size, float* __restrict__ C, int64_t N,
     int64_t Kp, int64_t ldc, int nM, int nN, int kst, int64_t slice_stride) {
@@ -536,6 +543,7 @@ __global__ __launch_bounds__(256, 1) void k_gemm_split_gen(
   const float inv_h = gtab[kGTab].x;
   GenB<false> gb;
   gb.pix = pix + kz / 2;
+  gb.pr2 = pr2 + kz / 2;
   gb.row = threadIdx.x;
   gb.nk = g.nk;
   {
@@ -556,9 +564,10 @@ __global__ __launch_bounds__(256, 1) void k_gemm_split_gen(
   __syncthreads();
   const int wm = g.wave >> 1, wn = g.wave & 1;
   f32x16 acc[4][4];
-  // a frame whose outermost pixel leaves the table sends its wave to the direct path
-  if (__any(r2max * gb.kdx > (float)kGTab)) {
-    const GenB<true> gd{gb.pix, gb.a, gb.b, gb.kdx, gb.kd2, gb.rel, gb.row, gb.nk};
+  // a frame whose outermost pixel comes within half a piece of the table's end sends its
+  // wave to the direct path
+  if (__any(r2max * gb.kdx >= (float)kGTab - 0.5f)) {
+    const GenB<true> gd{gb.pix, gb.pr2, gb.a, gb.b, gb.kdx, gb.kd2, gb.rel, gb.row, gb.nk};
     gen_loop(g, as, gd, acc, wm, wn);
   } else {
     gen_loop(g, as, gb, acc, wm, wn);
@@ -725,7 +734,8 @@ int xterm_sphere_table(double rel, float4* table, hipStream_t stream) {
 // Split-BF16 GEMM whose B rows are the reference terms of frames d0 .. d0 + N - 1, generated
 // in the kernel (pixels packed by the caller, table from xterm_sphere_table).
 int xterm_gemm_split_gen(const __bf16* ah, const __bf16* al, const uint32_t* pix,
-                         const float4* table, double r2max, double rel, const double* sx,
+                         const float* pr2, const float4* table, double r2max, double rel,
+                         const double* sx,
                          const double* sy, const double* dia, int64_t d0, int size, float* C,
                          int64_t Mpad, int64_t N, int64_t Kp, int64_t ldc, int64_t slice_stride,
                          hipStream_t stream) {
@@ -736,7 +746,7 @@ int xterm_gemm_split_gen(const __bf16* ah, const __bf16* al, const uint32_t* pix
   const int nM = (int)(Mpad / kTM), nN = (int)ceil_div(N, kTN);
   const int T8 = (nM * nN + 7) / 8;
   hipLaunchKernelGGL(k_gemm_split_gen, dim3(8 * T8, S), dim3(256), 0, stream, ah, al, pix,
-                     table, (float)r2max, (float)rel, sx, sy, dia, d0, size, C, N, Kp, ldc, nM,
+                     pr2, table, (float)r2max, (float)rel, sx, sy, dia, d0, size, C, N, Kp, ldc, nM,
                      nN, kSliceStages, slice_stride);
   RSP_LAUNCH_CHECK("k_gemm_split_gen");
   return kOk;

@@ -34,13 +34,15 @@ int xterm_gemm_split(const __bf16* ah, const __bf16* al, const __bf16* bh, const
 
 // The rotation form's split GEMM with B = the reference rows R_d(j) generated in the kernel
 // (B is never stored): frames d0 .. d0 + N - 1 of the arrays sx, sy, dia; pix[j] packs scored
-// pixel j of the K order as bf16 (x - c) | bf16 (y - c) << 16, with Kp / 2 entries (zero past
-// the last pixel); r2max the largest (x - c)^2 + (y - c)^2 of the scored pixels; table from
+// pixel j of the K order as bf16 (x - c) | bf16 (y - c) << 16 and pr2[j] its (x - c)^2 +
+// (y - c)^2, with Kp / 2 entries (zero past the last pixel); r2max the largest of them; table
+// from
 // xterm_sphere_table (xterm_gen_table_len() float4, built once per rel).
 int xterm_gen_table_len();
 int xterm_sphere_table(double rel, float4* table, hipStream_t stream);
 int xterm_gemm_split_gen(const __bf16* ah, const __bf16* al, const uint32_t* pix,
-                         const float4* table, double r2max, double rel, const double* sx,
+                         const float* pr2, const float4* table, double r2max, double rel,
+                         const double* sx,
                          const double* sy, const double* dia, int64_t d0, int size, float* C,
                          int64_t Mpad, int64_t N, int64_t Kp, int64_t ldc, int64_t slice_stride,
                          hipStream_t stream);
