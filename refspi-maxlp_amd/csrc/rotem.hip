@@ -153,10 +153,12 @@ __global__ __launch_bounds__(256) void k_rot_slices(
     if constexpr (MODE == kFP64) {
       ((double2*)OT)[(int64_t)jj * Hs + h] = v;
     } else {
-      // FP32 hi + lo: O = hi + lo to about 2^-48 relative (see k_rot_lt)
+      // FP32 hi and the remainders as bf16 (12 bytes): O = hi + lo to about 2^-33 relative
+      // (see k_rot_lt)
       const float hr = (float)v.x, hi = (float)v.y;
-      ((float4*)OT)[(int64_t)jj * Hs + h] =
-          make_float4(hr, hi, (float)(v.x - (double)hr), (float)(v.y - (double)hi));
+      const bf16x2 lo = {(__bf16)(float)(v.x - (double)hr), (__bf16)(float)(v.y - (double)hi)};
+      ((uint3*)OT)[(int64_t)jj * Hs + h] =
+          make_uint3(__float_as_uint(hr), __float_as_uint(hi), __builtin_bit_cast(uint32_t, lo));
     }
   }
 }
@@ -340,8 +342,9 @@ __global__ __launch_bounds__(256) void k_rot_lt(
     const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ dia, const double* __restrict__ scales, int64_t d0, int64_t d1,
     double* __restrict__ llb) {
-  // OT entries: FP64 (re, im), or FP32 (re_hi, im_hi, re_lo, im_lo)
-  typedef typename std::conditional<sizeof(T) == 4, float4, double2>::type T2;
+  // OT entries: FP64 (re, im), or FP32 (re_hi, im_hi) with the remainders as bf16 (re_lo in
+  // the low half of the third word, im_lo in the high half)
+  typedef typename std::conditional<sizeof(T) == 4, uint3, double2>::type T2;
   __shared__ int32_t s_j[4][64];
   __shared__ T s_k[4][64], s_rr[4][64], s_ri[4][64], s_rrl[4][64], s_ril[4][64];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -369,6 +372,9 @@ __global__ __launch_bounds__(256) void k_rot_lt(
     }
     __builtin_amdgcn_wave_barrier();
     const int n = (int)min((int64_t)64, o1 - base);
+    // eight photons' OT reads in flight per wave (the reads come from the MALL: latency, not
+    // bandwidth, bounds this loop)
+#pragma unroll 8
     for (int q = 0; q < n; ++q) {
       const T2 o = OT[(int64_t)s_j[w][q] * Hs];
       const T kq = s_k[w][q], rr = s_rr[w][q], ri = s_ri[w][q];
@@ -376,8 +382,8 @@ __global__ __launch_bounds__(256) void k_rot_lt(
       if constexpr (sizeof(T) == 4) {
         // the hi parts cancel exactly when O ~ -R (Sterbenz); the lo parts restore the
         // digits the FP32 roundings of O and R dropped
-        zr = (o.x + rr) + (o.z + s_rrl[w][q]);
-        zi = (o.y + ri) + (o.w + s_ril[w][q]);
+        zr = (__uint_as_float(o.x) + rr) + (__uint_as_float(o.z << 16) + s_rrl[w][q]);
+        zi = (__uint_as_float(o.y) + ri) + (__uint_as_float(o.z & 0xffff0000u) + s_ril[w][q]);
       } else {
         zr = o.x + rr;
         zi = o.y + ri;
@@ -954,6 +960,10 @@ struct rsp_rot {
   // timing
   bool timing = false;
   hipEvent_t ev[8] = {};
+  // side stream (default priority) for |R_d|^2 beside the split GEMM with generated rows,
+  // joined by events: [0] chunk start on the caller's stream, [1] norms done
+  hipStream_t side = nullptr;
+  hipEvent_t sev[2] = {};
   double t_ms[5] = {0., 0., 0., 0., 0.};
 };
 
@@ -1012,7 +1022,8 @@ int ensure_work(rsp_rot* p, int64_t chunk) {
   // no B buffer when the split GEMM generates its reference rows
   const int64_t Bpad = stores_b(p) ? xterm_pad_n(chunk) : 1;
   const int64_t ntiles = ceil_div(p->K, 64);
-  const int64_t ot_bytes = (int64_t)p->K * p->Hs * 16;
+  // OT entries: FP64 re/im (16 B), or FP32 hi re/im + bf16 lo re/im (12 B)
+  const int64_t ot_bytes = (int64_t)p->K * p->Hs * (p->mode == kFP64 ? 16 : 12);
   const int xs = p->mode == kSplit ? xterm_split_k(p->Kp) : 1;
   int rc;
   char** A = reinterpret_cast<char**>(&p->d_A);
@@ -1160,6 +1171,13 @@ int ensure_events(rsp_rot* p) {
   return kOk;
 }
 
+int ensure_side(rsp_rot* p) {
+  if (!p->side) RSP_HIP(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+  for (auto& e : p->sev)
+    if (!e) RSP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return kOk;
+}
+
 // E-step over the frames of `f` (per-frame arrays indexed like f's frames), in GEMM chunks of
 // p->chunk_alloc frames.  The slices of the forward mode must be current.
 int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, const double* dia,
@@ -1170,13 +1188,20 @@ int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, cons
   const bool gen = !stores_b(p);
   int rc = kOk;
   if (p->timing && (rc = ensure_events(p))) return rc;
+  if (gen && (rc = ensure_side(p))) return rc;
   for (int64_t d0 = 0; d0 < f.n; d0 += chunk) {
     const int64_t n = std::min(chunk, f.n - d0);
     if (p->timing) RSP_HIP(hipEventRecord(p->ev[2], st));
     if (gen) {
-      hipLaunchKernelGGL(k_ref_norm, dim3((unsigned)n), dim3(256), 0, st, p->d_rad,
+      // |R_d|^2 on the side stream, beside the GEMM (it needs only the frames' diameters);
+      // it starts after everything queued on st so far (the previous chunk's log term has
+      // read normR) and the log term below waits for it
+      RSP_HIP(hipEventRecord(p->sev[0], st));
+      RSP_HIP(hipStreamWaitEvent(p->side, p->sev[0], 0));
+      hipLaunchKernelGGL(k_ref_norm, dim3((unsigned)n), dim3(256), 0, p->side, p->d_rad,
                          p->d_radcnt, p->nrad, p->rel, p->size, dia, d0, p->d_normR);
       RSP_LAUNCH_CHECK("k_ref_norm");
+      RSP_HIP(hipEventRecord(p->sev[1], p->side));
     } else if (p->mode == kFP32)
       rc = launch_ref_to<kFP32>(p, sx, sy, dia, d0, n, f.n, p->d_B, Bpad, p->d_normR, st);
     else if (p->mode == kSplit)
@@ -1204,6 +1229,7 @@ int run_estep(rsp_rot* p, const Csr& f, const double* sx, const double* sy, cons
     }
     if (rc) return rc;
     if (p->timing) RSP_HIP(hipEventRecord(p->ev[4], st));
+    if (gen) RSP_HIP(hipStreamWaitEvent(st, p->sev[1], 0));
     if (p->mode == kFP64)
       rc = launch_ll<double>(p, f, sx, sy, dia, scales, d0, d0 + n, ll, prob, logz, best, st);
     else
@@ -1940,6 +1966,9 @@ int rsp_rot_destroy(rsp_rot* p) {
   dfree(p->d_pk);
   for (hipEvent_t e : p->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : p->sev)
+    if (e) (void)hipEventDestroy(e);
+  if (p->side) (void)hipStreamDestroy(p->side);
   if (p->blas) rocblas_destroy_handle(p->blas);
   delete p;
   return kOk;

@@ -317,16 +317,27 @@ struct GenB {
   }
 
   __device__ __forceinline__ void prep(const float (&r2)[16], int kg, int hf, Prep& p) const {
+    prep2(r2[4 * kg + 2 * hf], r2[4 * kg + 2 * hf + 1], p);
+  }
+
+  // the first half of stage q (its squared radii by two scalar loads), issued one stage ahead
+  __device__ __forceinline__ void prep_first(int q, Prep& p) const {
+    const int64_t o = (int64_t)min(q, nk - 1) * (kBK / 2);
+    prep2(pr2[o], pr2[o + 1], p);
+  }
+
+  __device__ __forceinline__ void prep2(float r2a, float r2b, Prep& p) const {
+    const float r2[2] = {r2a, r2b};
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      const float x = r2[4 * kg + 2 * hf + u] * kdx;  // table coordinate v / h (>= 0)
+      const float x = r2[u] * kdx;  // table coordinate v / h (>= 0)
       // x < kGTab - 1/2 outside DIRECT waves: the piece is x's integer part, f its fraction
       const int i = DIRECT ? min((int)x, kGTab - 1) : (int)x;
       p.f[u] = DIRECT ? x - (float)i : __builtin_amdgcn_fractf(x);
       p.c[u] = s_gtab[i];
       if constexpr (DIRECT) {
         p.x[u] = x;
-        p.r2[u] = r2[4 * kg + 2 * hf + u];
+        p.r2[u] = r2[u];
       }
     }
   }
@@ -417,7 +428,8 @@ struct AStream {
 // the first step's third product, which carries no generation.
 template <int S, int SB, bool DIRECT>
 __device__ __forceinline__ void compute_gen(const SplitGemm& g, const GenB<DIRECT>& gb, int qn,
-                                            f32x16 (&acc)[4][4], int wm, int wn) {
+                                            f32x16 (&acc)[4][4], int wm, int wn,
+                                            typename GenB<DIRECT>::Prep& pa) {
   uint32_t w[16];
   float r2[16];
   gb.words(qn, w, r2);
@@ -430,13 +442,14 @@ __device__ __forceinline__ void compute_gen(const SplitGemm& g, const GenB<DIREC
                  "+v"(f0.bl[0]), "+v"(f0.bl[1]), "+v"(f0.bl[2]), "+v"(f0.bl[3])
                :
                : "memory");
-  typename GenB<DIRECT>::Prep pa, pb;
-  gb.prep(r2, 0, 0, pa);
+  // pa holds the first half of stage qn (its table reads issued during the previous stage)
+  typename GenB<DIRECT>::Prep pb;
   // a region: 8 MFMAs, the table reads of the next half and the VALU work of this one
   // between them
 #define RSP_GEN_REGION(PR, I0, F, KG, HF, PN, KGN, HFN, PC)                  \
   SplitGemm::mma_rows<PR, I0>(acc, F);                                      \
   if (KGN >= 0) gb.prep(r2, KGN, HFN, PN);                                  \
+  else gb.prep_first(qn + 1, PN);                                           \
   gb.template finish<SB>(w, KG, HF, PC);                                    \
   for (int u_ = 0; u_ < 8; ++u_) {                                          \
     __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* one MFMA */       \
@@ -458,7 +471,7 @@ __device__ __forceinline__ void compute_gen(const SplitGemm& g, const GenB<DIREC
   RSP_GEN_REGION(0, 0, f1, 2, 0, pb, 2, 1, pa)
   RSP_GEN_REGION(0, 2, f1, 2, 1, pa, 3, 0, pb)
   RSP_GEN_REGION(1, 0, f1, 3, 0, pb, 3, 1, pa)
-  RSP_GEN_REGION(1, 2, f1, 3, 1, pa, -1, 0, pb)
+  RSP_GEN_REGION(1, 2, f1, 3, 1, pa, -1, 0, pb)  // the next stage's first half into pa
 #undef RSP_GEN_REGION
   SplitGemm::mma<2>(acc, f1);
 }
@@ -485,6 +498,8 @@ __device__ __forceinline__ void gen_loop(const SplitGemm& g, const AStream& as,
       gb.template half<0>(w, r2, kg, 1);
     }
   }
+  typename GenB<DIRECT>::Prep pa;
+  gb.prep_first(1, pa);
   for (int q = 0; q < g.nk; q += 2) {
     // vmcnt = 0: this wave's A loads of stage q have landed; lgkmcnt = 0: its B-tile writes
     // of stage q and its fragment reads of stage q-1 are done (the buffers of stage q+1
@@ -493,12 +508,12 @@ __device__ __forceinline__ void gen_loop(const SplitGemm& g, const AStream& as,
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
     as.issue<1>(q + 1);
-    compute_gen<0, 1>(g, gb, q + 1, acc, wm, wn);
+    compute_gen<0, 1>(g, gb, q + 1, acc, wm, wn, pa);
     __builtin_amdgcn_s_waitcnt(0x070);
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
     as.issue<0>(q + 2);
-    if (q + 1 < g.nk) compute_gen<1, 0>(g, gb, q + 2, acc, wm, wn);
+    if (q + 1 < g.nk) compute_gen<1, 0>(g, gb, q + 2, acc, wm, wn, pa);
   }
 }
 
