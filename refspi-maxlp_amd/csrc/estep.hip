@@ -47,6 +47,9 @@ constexpr int kPStride = kChunks * 256;
 constexpr int kPartMax = kPStride - 32;
 constexpr int kNBuf = 3;
 constexpr uint32_t kRebase = 2 * kPStride * 4 - 64512;  // third buffer's base - offset
+// word 3 of a raw buffer resource on gfx9-family GPUs: 32-bit data format, no swizzle
+// (offsets are plain byte offsets checked against the record count)
+constexpr int kBufWord3 = 0x00020000;
 
 struct Top3 {
   double v1, v2, v3;
@@ -322,6 +325,23 @@ __device__ __forceinline__ void load_block(const char* lds, const uint32_t (&a)[
   }
 }
 
+// Holds the block's values until here: an empty asm that reads and rewrites them, placed
+// after the next block's reads are issued, so that the wait for this block's reads comes
+// after that issue (lgkmcnt(N), the next block still in flight) and the compiler cannot
+// start this block's sums -- and so wait for it -- before the next block is issued.
+// HOLD = false (slot classes near the register limit, where holding both blocks' values
+// would spill): the compiler places the waits, per value.
+template <int N, bool HOLD>
+__device__ __forceinline__ void hold_block(float (&w)[8]) {
+  if constexpr (!HOLD)
+    return;
+  else if constexpr (N == 8)
+    asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3]), "+v"(w[4]), "+v"(w[5]),
+                 "+v"(w[6]), "+v"(w[7])::"memory");
+  else
+    asm volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3])::"memory");
+}
+
 // slots in the block starting at K of the range ending at K1 (8, or the 4 of a tail)
 template <int K, int K1>
 constexpr int block_len() {
@@ -333,7 +353,7 @@ constexpr int block_len() {
 // reads at once.  The empty asm statements pin the order: each block's sum is complete
 // before the next block's reads are issued (memory clobber), and cannot sink below them.
 // PEND: the FP32 sum of the previous block is waiting in `pend` for its pair.
-template <int K, int K1, int P, int B, bool REB, bool PEND>
+template <int K, int K1, int P, int B, bool REB, bool PEND, bool HOLD>
 __device__ __forceinline__ void gather_pipe(const char* lds, const uint32_t (&a)[P],
                                             float (&v)[2][8], double (&acc)[4], uint32_t rb,
                                             float pend) {
@@ -341,6 +361,7 @@ __device__ __forceinline__ void gather_pipe(const char* lds, const uint32_t (&a)
     constexpr int N = block_len<K, K1>();
     static_assert(N == 8 || N == 4, "parts hold a multiple of 4 slots");
     if constexpr (K + N < K1) load_block<K + N, block_len<K + N, K1>(), P, REB>(lds, a, v[B ^ 1], rb);
+    hold_block<N, HOLD>(v[B]);
     const float* w = v[B];
     float s8;
     if constexpr (N == 8)
@@ -350,16 +371,16 @@ __device__ __forceinline__ void gather_pipe(const char* lds, const uint32_t (&a)
     asm volatile("" : "+v"(s8) : : "memory");
     if constexpr (PEND) {
       acc[(K >> 4) & 3] += (double)(pend + s8);
-      gather_pipe<K + N, K1, P, B ^ 1, REB, false>(lds, a, v, acc, rb, 0.f);
+      gather_pipe<K + N, K1, P, B ^ 1, REB, false, HOLD>(lds, a, v, acc, rb, 0.f);
     } else if constexpr (K + N >= K1) {
       acc[(K >> 4) & 3] += (double)s8;  // last block of the part, unpaired
     } else {
-      gather_pipe<K + N, K1, P, B ^ 1, REB, true>(lds, a, v, acc, rb, s8);
+      gather_pipe<K + N, K1, P, B ^ 1, REB, true, HOLD>(lds, a, v, acc, rb, s8);
     }
   }
 }
 
-template <int K0, int K1, int P, bool REB>
+template <int K0, int K1, int P, bool REB, bool HOLD>
 __device__ __forceinline__ void gather_range(const char* lds, const uint32_t (&a)[P],
                                              double (&acc)[4]) {
   if constexpr (K0 < K1) {
@@ -369,7 +390,7 @@ __device__ __forceinline__ void gather_range(const char* lds, const uint32_t (&a
     if constexpr (REB) asm volatile("" : "=s"(rb) : "0"(kRebase));
     float v[2][8];
     load_block<K0, block_len<K0, K1>(), P, REB>(lds, a, v[0], rb);
-    gather_pipe<K0, K1, P, 0, REB, false>(lds, a, v, acc, rb, 0.f);
+    gather_pipe<K0, K1, P, 0, REB, false, HOLD>(lds, a, v, acc, rb, 0.f);
   }
 }
 
@@ -405,7 +426,7 @@ __device__ __forceinline__ void load_block2(const char* lds, const uint32_t (&a)
   }
 }
 
-template <int S, int PP, int K, int P, int B, bool REB, bool PEND>
+template <int S, int PP, int K, int P, int B, bool REB, bool PEND, bool HOLD>
 __device__ __forceinline__ void gather_pipe2(const char* lds, const uint32_t (&a)[P],
                                              float (&v)[2][8], double (&acc)[4], uint32_t rb,
                                              float pend, const uint32_t (&ext)[2]) {
@@ -416,6 +437,7 @@ __device__ __forceinline__ void gather_pipe2(const char* lds, const uint32_t (&a
     constexpr bool last = (K + N) % S == 0;  // the frame's last block
     if constexpr (K + N < 2 * S)
       load_block2<S, PP, K + N, block_len2<S, K + N>(), P, REB>(lds, a, v[B ^ 1], rb, ext);
+    hold_block<N, HOLD>(v[B]);
     const float* w = v[B];
     float s8;
     if constexpr (N == 8)
@@ -426,17 +448,17 @@ __device__ __forceinline__ void gather_pipe2(const char* lds, const uint32_t (&a
     constexpr int ai = 2 * f + (((K % S) >> 4) & 1);
     if constexpr (PEND) {
       acc[ai] += (double)(pend + s8);
-      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, false>(lds, a, v, acc, rb, 0.f, ext);
+      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, false, HOLD>(lds, a, v, acc, rb, 0.f, ext);
     } else if constexpr (last) {
       acc[ai] += (double)s8;
-      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, false>(lds, a, v, acc, rb, 0.f, ext);
+      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, false, HOLD>(lds, a, v, acc, rb, 0.f, ext);
     } else {
-      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, true>(lds, a, v, acc, rb, s8, ext);
+      gather_pipe2<S, PP, K + N, P, B ^ 1, REB, true, HOLD>(lds, a, v, acc, rb, s8, ext);
     }
   }
 }
 
-template <int S, int PP, int P, bool REB>
+template <int S, int PP, int P, bool REB, bool HOLD>
 __device__ __forceinline__ void gather_range2(const char* lds, const uint32_t (&a)[P],
                                               double (&acc)[4]) {
   // rb = 1 (opaque): the third buffer's addresses carry bit 16
@@ -448,7 +470,7 @@ __device__ __forceinline__ void gather_range2(const char* lds, const uint32_t (&
   asm volatile("" : "=s"(ext[1]) : "0"(16u));
   float v[2][8];
   load_block2<S, PP, 0, block_len2<S, 0>(), P, REB>(lds, a, v[0], rb, ext);
-  gather_pipe2<S, PP, 0, P, 0, REB, false>(lds, a, v, acc, rb, 0.f, ext);
+  gather_pipe2<S, PP, 0, P, 0, REB, false, HOLD>(lds, a, v, acc, rb, 0.f, ext);
 }
 
 template <int K0, int P>
@@ -485,6 +507,12 @@ template <int S, int NP, int NW, int L, int X, int F>
 struct GatherStep {
   static constexpr int P = S * NP;
   static_assert(F == 1 || (F == 2 && X == 1), "two frames per lane: whole frames only");
+  // VGPRs per lane at NW / 4 waves per SIMD (512 per SIMD lane, granules of 8), and whether
+  // the slot registers leave room to hold a block's values past the next block's issue
+  // (about 49 working registers with one frame per lane, 64 with two, held values
+  // included; measured on the instantiations: beyond this the held values spill)
+  static constexpr int kVgprBudget = (512 / (NW / 4)) & ~7;
+  static constexpr bool kHold = P + (F == 2 ? 64 : 49) <= kVgprBudget;
   const float* tables;
   int TS, hb, h0, h1, nq, R, wid, lane;
   double scale[F];  // frame f of the lane (F = 2: low / high halves of the slot registers)
@@ -509,27 +537,30 @@ struct GatherStep {
     const float* src = tables + (int64_t)(h - hb) * TS + (int64_t)p * kPStride;
     // addresses as a wave-uniform base + one 32-bit lane offset (scalar-base loads: no
     // per-load address registers)
-    const char* sb = reinterpret_cast<const char*>(src);
+    // Buffer loads (MUBUF) rather than global ones: the compiler counts a global (FLAT)
+    // load into LDS as an LDS access as well, and while one is pending it waits for all LDS
+    // reads at every wait (lgkmcnt(0)); a buffer load is counted by vmcnt alone.  The
+    // resource spans the part exactly (50,176 B: every load below stays inside it).
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(src), (short)0, kPStride * 4, kBufWord3);
     const unsigned l16 = NP > 2 ? (unsigned)lane * 16u : (PP == 0 ? off0 : off1),
                    l4 = (unsigned)lane * 4u;
 #pragma unroll
     for (int u = 0; u < L - 1; ++u) {
       const int c = wid + u * NW;
-      __builtin_amdgcn_global_load_lds(
-          (const __attribute__((address_space(1))) void*)(sb + c * 1024 + l16),
-          (__attribute__((address_space(3))) void*)(dst + c * 256), 16, 0, 0);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rs, (__attribute__((address_space(3))) void*)(dst + c * 256), 16, l16, c * 1024, 0,
+          0);
     }
     const int qo = (kChunks - 1) * 256 + (wid & 3) * 64;
-    __builtin_amdgcn_global_load_lds(
-        (const __attribute__((address_space(1))) void*)(sb + qo * 4 + l4),
-        (__attribute__((address_space(3))) void*)(dst + qo), 4, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        rs, (__attribute__((address_space(3))) void*)(dst + qo), 4, l4, qo * 4, 0, 0);
   }
 
   // static step QS (0 .. 3*NP-1) of group g
   template <int QS>
-  __device__ __forceinline__ void step(int g, const uint32_t (&a)[P],
-                                       const double* __restrict__ init, double (&acc)[4],
-                                       Top3 (&top)[F]) const {
+  __device__ __forceinline__ void step(int g, const uint32_t (&a)[P], const double (&ini)[3],
+                                       double (&acc)[4], Top3 (&top)[F]) const {
     constexpr int PP = QS % NP;
     const int q = g * 3 * NP + QS;
     const int h = h0 + g * 3 + QS / NP;
@@ -548,12 +579,12 @@ struct GatherStep {
     issue<(QS + 2) % NP>(q + 2, part_buf<(QS + 2) % 3>());
     if constexpr (F == 2) {
       if constexpr (PP == 0) {
-        const double in = init[min(h, h1 - 1) / R];
+        const double in = ini[QS / NP];
         acc[0] = in * scale[0];
         acc[2] = in * scale[1];
         acc[1] = acc[3] = 0.;
       }
-      gather_range2<S, PP, P, QS % 3 == 2>(reinterpret_cast<const char*>(part_buf<QS % 3>()),
+      gather_range2<S, PP, P, QS % 3 == 2, kHold>(reinterpret_cast<const char*>(part_buf<QS % 3>()),
                                            a, acc);
       if constexpr (PP == NP - 1) {
         const double t0 = acc[0] + acc[1], t1 = acc[2] + acc[3];
@@ -563,10 +594,10 @@ struct GatherStep {
       return;
     }
     if constexpr (PP == 0) {
-      acc[0] = init[min(h, h1 - 1) / R] * scale[0];
+      acc[0] = ini[QS / NP] * scale[0];
       acc[1] = acc[2] = acc[3] = 0.;
     }
-    gather_range<PP * S, PP * S + S, P, QS % 3 == 2>(
+    gather_range<PP * S, PP * S + S, P, QS % 3 == 2, kHold>(
         reinterpret_cast<const char*>(part_buf<QS % 3>()), a, acc);
     if constexpr (PP == NP - 1) {
       double tot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
@@ -586,13 +617,29 @@ struct GatherStep {
   }
 
   template <int QS>
+  __device__ __forceinline__ void group_steps(int g, const uint32_t (&a)[P],
+                                              const double (&ini)[3], double (&acc)[4],
+                                              Top3 (&top)[F]) const {
+    if constexpr (QS < 3 * NP) {
+      step<QS>(g, a, ini, acc, top);
+      group_steps<QS + 1>(g, a, ini, acc, top);
+    }
+  }
+
+  // The group's three init terms (init[s] of its hypotheses) are read at its start and
+  // moved to VGPRs at once: a scalar load shares lgkmcnt with the LDS reads and may complete
+  // out of order, so while one is pending every wait for the walk's reads drains all of
+  // them.
   __device__ __forceinline__ void group(int g, const uint32_t (&a)[P],
                                         const double* __restrict__ init, double (&acc)[4],
                                         Top3 (&top)[F]) const {
-    if constexpr (QS < 3 * NP) {
-      step<QS>(g, a, init, acc, top);
-      group<QS + 1>(g, a, init, acc, top);
+    double ini[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      ini[t] = init[min(h0 + g * 3 + t, h1 - 1) / R];
+      asm volatile("" : "+v"(ini[t]));
     }
+    group_steps<0>(g, a, ini, acc, top);
   }
 };
 
@@ -640,7 +687,7 @@ __global__ __launch_bounds__(64 * NW) void k_gather(
   st.template issue<1 % NP>(1, g_buf1);
   double acc[4];
   const int ngroups = (h1 - h0 + 2) / 3;
-  for (int g = 0; g < ngroups; ++g) st.template group<0>(g, a, init, acc, top);
+  for (int g = 0; g < ngroups; ++g) st.group(g, a, init, acc, top);
   // no direct-to-LDS write may still be in flight when the workgroup's LDS is released
   __builtin_amdgcn_s_waitcnt(0xF70);
 #pragma unroll
