@@ -509,10 +509,11 @@ struct GatherStep {
   static_assert(F == 1 || (F == 2 && X == 1), "two frames per lane: whole frames only");
   // VGPRs per lane at NW / 4 waves per SIMD (512 per SIMD lane, granules of 8), and whether
   // the slot registers leave room to hold a block's values past the next block's issue
-  // (about 49 working registers with one frame per lane, 64 with two, held values
-  // included; measured on the instantiations: beyond this the held values spill)
+  // (about 40 working registers with one frame per lane, 56 with two, held values
+  // included; not for frames split over lanes, whose shuffles need more; measured on the
+  // instantiations: beyond this the held values spill)
   static constexpr int kVgprBudget = (512 / (NW / 4)) & ~7;
-  static constexpr bool kHold = P + (F == 2 ? 64 : 49) <= kVgprBudget;
+  static constexpr bool kHold = X == 1 && P + (F == 2 ? 56 : 40) <= kVgprBudget;
   const float* tables;
   int TS, hb, h0, h1, nq, R, wid, lane;
   double scale[F];  // frame f of the lane (F = 2: low / high halves of the slot registers)
@@ -626,10 +627,10 @@ struct GatherStep {
     }
   }
 
-  // The group's three init terms (init[s] of its hypotheses) are read at its start and
-  // moved to VGPRs at once: a scalar load shares lgkmcnt with the LDS reads and may complete
-  // out of order, so while one is pending every wait for the walk's reads drains all of
-  // them.
+  // The group's three init terms (init[s] of its hypotheses, wave-uniform) are read at its
+  // start and waited for at once (an empty asm reads them; they stay in SGPRs): a scalar
+  // load shares lgkmcnt with the LDS reads and may complete out of order, so while one is
+  // pending every wait for the walk's reads drains all of them.
   __device__ __forceinline__ void group(int g, const uint32_t (&a)[P],
                                         const double* __restrict__ init, double (&acc)[4],
                                         Top3 (&top)[F]) const {
@@ -637,7 +638,7 @@ struct GatherStep {
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
       ini[t] = init[min(h0 + g * 3 + t, h1 - 1) / R];
-      asm volatile("" : "+v"(ini[t]));
+      asm volatile("" : "+s"(ini[t]));
     }
     group_steps<0>(g, a, ini, acc, top);
   }
