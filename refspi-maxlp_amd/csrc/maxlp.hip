@@ -307,8 +307,9 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
   // Candidates of this thread.  Search: pattern column iy = threadIdx % 33 and rows
   // ix = 5 * (threadIdx / 33) + u (231 threads cover the 33 x 33 pattern; ix >= 33 and
   // threads >= 231 hold none), so the imaginary part -- and im^2 below -- is shared by the
-  // thread's candidates: 22 FP64 operations per entry instead of 30, each candidate's
-  // |c + R|^2 rounded exactly as before.  Probe: the caller's candidates threadIdx + 256 u.
+  // thread's candidates: 17 FP64 operations per entry (2 shared, then an add, a fused
+  // multiply-add and the product's multiply per candidate).  Probe: the caller's
+  // candidates threadIdx + 256 u.
   int cidx[kPerThread];
 #pragma unroll
   for (int u = 0; u < kPerThread; ++u) {
@@ -341,65 +342,80 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       prod[u] = 1.;
       ex[u] = 0;
     }
-    for (int64_t cb = e0; cb < e1; cb += kChunk) {
-      const int n = (int)min((int64_t)kChunk, e1 - cb);
-      __syncthreads();
-      for (int i = threadIdx.x; i < n; i += kThreads) sh[i] = ent[cb + i];
-      __syncthreads();
-      // entry i+1 is read from LDS before the arithmetic of entry i, so that the read's
-      // latency is not waited for on every entry
-      Entry En = sh[0];
-      for (int i = 0; i < n; ++i) {
-        const Entry E = En;
-        if (i + 1 < n) En = sh[i + 1];
-        const int K = (int)E.k;
-        // im^2 of the shared imaginary part (search), per candidate (probe)
-        double b0 = 0.;
-        if (!kProbe) {
-          const double im = ci[0] + E.ri;
-          b0 = im * im;
+    // One entry: prod[u] *= |c_u + R|^2 (K times for a count K > 1), in entry order.
+    auto factor = [&](double rr, double ri, int K) {
+      // im^2 of the shared imaginary part (search), per candidate (probe)
+      double b0 = 0.;
+      if (!kProbe) {
+        const double im = ci[0] + ri;
+        b0 = im * im;
+      }
+#pragma unroll
+      for (int u = 0; u < kPerThread; ++u) {
+        const double re = cr[u] + rr;
+        double b = b0;
+        if (kProbe) {
+          const double im = ci[kProbe ? u : 0] + ri;
+          b = im * im;
         }
-        if (K == 1) {  // most entries: straight-line code over the candidates
+        // |c + R|^2 as one fused multiply-add (the re^2 product unrounded; the VALU issue
+        // bounds this loop, and the fusion takes a quarter of its instructions)
+        const double a = fma(re, re, b);
+        double pr = prod[u] * a;
+        for (int k = 1; k < K; ++k) pr = pr * a;  // 2 <= K <= 8: k_pull splits larger counts
+        prod[u] = pr;
+      }
+    };
+    auto renorm = [&]() {
 #pragma unroll
-          for (int u = 0; u < kPerThread; ++u) {
-            const double re = cr[u] + E.rr;
-            double b = b0;
-            if (kProbe) {
-              const double im = ci[kProbe ? u : 0] + E.ri;
-              b = im * im;
-            }
-            prod[u] = prod[u] * (re * re + b);
-          }
-        } else {
-          // 2 <= K <= 8 (k_pull splits larger counts): a^K directly, renormalised below
+      for (int u = 0; u < kPerThread; ++u) {
+        int e;
+        prod[u] = frexp(prod[u], &e);
+        ex[u] += e;
+      }
+    };
+    // The entries are the same for every thread: they are read by scalar loads (uniform
+    // addresses, into SGPRs), four at a time with the next four in flight, and used as
+    // scalar operands -- no LDS staging, no barriers.  Renormalisation every 8 factors
+    // (entry j of the pixel with j % 8 == 7) and after every K > 1 entry: at most 8 factors
+    // between renormalisations, so the products stay within FP64's normal range whenever
+    // |f_c + R_d|^2 is in [2^-120, 2^120] (scaling by powers of two is exact, so the period
+    // does not change the product's rounding; an exact zero stays zero, giving -inf as log 0
+    // does in the reference).
+    const int64_t n = e1 - e0;
+    const int64_t n4 = n & ~(int64_t)3;
+    Entry nx[4];
+    if (n4 > 0) {
 #pragma unroll
-          for (int u = 0; u < kPerThread; ++u) {
-            const double re = cr[u] + E.rr;
-            double b = b0;
-            if (kProbe) {
-              const double im = ci[kProbe ? u : 0] + E.ri;
-              b = im * im;
-            }
-            const double a = re * re + b;
-            double pr = prod[u] * a;
-            for (int k = 1; k < K; ++k) pr = pr * a;
-            prod[u] = pr;
-          }
-        }
-        // renormalise every 8 factors and after every K > 1 entry: at most 8 factors between
-        // renormalisations, so the products stay within FP64's normal range whenever
-        // |f_c + R_d|^2 is in [2^-120, 2^120] (scaling by powers of two is exact, so the
-        // period does not change the product's rounding; an exact zero stays zero, giving
-        // -inf as log 0 does in the reference)
-        if ((i & 7) == 7 || K > 1) {
+      for (int j = 0; j < 4; ++j) nx[j] = ent[e0 + j];
+    }
+    for (int64_t j0 = 0; j0 < n4; j0 += 4) {
+      Entry cu[4];
 #pragma unroll
-          for (int u = 0; u < kPerThread; ++u) {
-            int e;
-            prod[u] = frexp(prod[u], &e);
-            ex[u] += e;
-          }
+      for (int j = 0; j < 4; ++j) cu[j] = nx[j];
+      if (j0 + 4 < n4) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) nx[j] = ent[e0 + j0 + 4 + j];
+      }
+      if (cu[0].k == 1. && cu[1].k == 1. && cu[2].k == 1. && cu[3].k == 1.) {
+        // four single-photon entries, straight-line (the common case)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) factor(cu[j].rr, cu[j].ri, 1);
+        if (j0 & 4) renorm();  // the block's last entry is j0 + 3, 7 mod 8
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int K = (int)cu[j].k;
+          factor(cu[j].rr, cu[j].ri, K);
+          if (((j0 + j) & 7) == 7 || K > 1) renorm();
         }
       }
+    }
+    for (int64_t j = n4; j < n; ++j) {
+      const Entry E = ent[e0 + j];
+      const int K = (int)E.k;
+      factor(E.rr, E.ri, K);
+      if ((j & 7) == 7 || K > 1) renorm();
     }
     // candidate values and the local first-occurrence argmax
     double bv = -INFINITY;
