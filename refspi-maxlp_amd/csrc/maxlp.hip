@@ -34,14 +34,22 @@ namespace rsp {
 
 constexpr int kCand = 1089;       // 33 x 33 pattern (maxLP.py:45-48, nmax = 8)
 constexpr int kCentre = 544;      // pattern index of 0 + 0j
-// 256 threads x 5 candidates (1280 >= 1089) per pixel: measured on the 185^2 workload
+// 256 threads x 5 candidates (1280 >= 1089) per team: measured on the 185^2 workload
 // (100k frames), 0.20 s per MaxLP pass against 0.27 s for 128 x 9 and 0.42 s for 64 x 18 --
-// more waves per pixel hide the FP64 latency and stage the entries faster than more
-// candidates per thread save on per-entry work
-constexpr int kThreads = 256;
+// more waves per pixel hide the FP64 latency better than more candidates per thread save
+// on per-entry work
+constexpr int kTeam = 256;        // threads holding the candidates of one pixel
 constexpr int kPerThread = 5;     // candidates per thread (7 x 5 rows x 33 columns >= 33 x 33)
+// Entry groups: a pixel's workgroup is kGroups teams, each taking a contiguous quarter of the
+// pixel's entries for all candidates; their partial products are combined in LDS every
+// round.  Four waves per SIMD on one pixel keep the FP64 pipelines issuing where a single
+// wave per SIMD waits on its dependency chains: the heaviest pixels (tens of thousands of
+// entries x 10 rounds) otherwise run alone at the end of the launch at a fraction of the
+// issue rate (185^2, 100k frames: CUs busy 40% of the launch, VALU 87% of the busy time).
+constexpr int kGroups = 4;
+constexpr int kThreads = kTeam * kGroups;
 constexpr int kWaves = kThreads / 64;
-constexpr int kChunk = 512;       // entries staged in LDS at a time
+constexpr int kTeamWaves = kTeam / 64;
 constexpr int kMaxPiece = 8;      // largest photon count of one model-space entry
 
 struct Entry {
@@ -269,10 +277,13 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
     const double* __restrict__ mom, int N, double rescale, double* __restrict__ model,
     double* __restrict__ resc_out, int probe_t, const double* __restrict__ probe_cand,
     int probe_n, double* __restrict__ probe_out, unsigned long long* __restrict__ counters) {
-  __shared__ Entry sh[kChunk];
-  __shared__ double wv[kThreads / 64];
-  __shared__ int wi[kThreads / 64];
+  // partial products of entry groups 1 .. kGroups-1 (normalised mantissa, exponent)
+  __shared__ double cpm[kGroups - 1][kPerThread][kTeam];
+  __shared__ int cpe[kGroups - 1][kPerThread][kTeam];
+  __shared__ double wv[kWaves];
+  __shared__ int wi[kWaves];
   __shared__ double skt;
+  const int ct = (int)threadIdx.x % kTeam, grp = (int)threadIdx.x / kTeam;
   const int li = kProbe ? 0 : (int)(uint32_t)order[blockIdx.x];
   const int t = kProbe ? probe_t : pixels[li];
   const int64_t e0 = seg_beg[t], e1 = seg_end[t];
@@ -304,23 +315,26 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
   double fr = kProbe ? 0. : model[2 * t], fi = kProbe ? 0. : model[2 * t + 1];
   double step = hypot(fr, fi) / 4.;
   int rounds = 0;  // candidate evaluation rounds (search steps evaluated)
-  // Candidates of this thread.  Search: pattern column iy = threadIdx % 33 and rows
-  // ix = 5 * (threadIdx / 33) + u (231 threads cover the 33 x 33 pattern; ix >= 33 and
-  // threads >= 231 hold none), so the imaginary part -- and im^2 below -- is shared by the
-  // thread's candidates: 17 FP64 operations per entry (2 shared, then an add, a fused
-  // multiply-add and the product's multiply per candidate).  Probe: the caller's
-  // candidates threadIdx + 256 u.
+  // Candidates of this thread (the same in every entry group).  Search: pattern column
+  // iy = ct % 33 and rows ix = 5 * (ct / 33) + u (231 threads of the team cover the 33 x 33
+  // pattern; ix >= 33 and threads >= 231 hold none), so the imaginary part -- and im^2
+  // below -- is shared by the thread's candidates: 17 FP64 operations per entry (2 shared,
+  // then an add, a fused multiply-add and the product's multiply per candidate).  Probe: the
+  // caller's candidates ct + 256 u.
   int cidx[kPerThread];
 #pragma unroll
   for (int u = 0; u < kPerThread; ++u) {
     if (kProbe) {
-      const int c = threadIdx.x + kThreads * u;
+      const int c = ct + kTeam * u;
       cidx[u] = c < probe_n ? c : -1;
     } else {
-      const int ix = 5 * (int)(threadIdx.x / 33) + u;
-      cidx[u] = ix < 33 ? ix * 33 + (int)(threadIdx.x % 33) : -1;
+      const int ix = 5 * (ct / 33) + u;
+      cidx[u] = ix < 33 ? ix * 33 + ct % 33 : -1;
     }
   }
+  // this group's entries: a contiguous slice of the pixel's, a multiple of 8 long
+  const int64_t glen = (((e1 - e0) + kGroups - 1) / kGroups + 7) & ~(int64_t)7;
+  const int64_t s0 = min(e1, e0 + grp * glen), s1 = min(e1, s0 + glen);
   for (int it = 0; it < (kProbe ? 1 : 10); ++it) {
     ++rounds;
     double cr[kPerThread], ci[kProbe ? kPerThread : 1], prod[kPerThread];
@@ -335,7 +349,7 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
         const double px = -8. + 0.5 * (double)(cc / 33);
         cr[u] = fr + px * step;
         if (u == 0) {
-          const double py = -8. + 0.5 * (double)(threadIdx.x % 33);
+          const double py = -8. + 0.5 * (double)(ct % 33);
           ci[0] = fi + py * step;
         }
       }
@@ -374,20 +388,20 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
         ex[u] += e;
       }
     };
-    // The entries are the same for every thread: they are read by scalar loads (uniform
-    // addresses, into SGPRs), four at a time with the next four in flight, and used as
-    // scalar operands -- no LDS staging, no barriers.  Renormalisation every 8 factors
-    // (entry j of the pixel with j % 8 == 7) and after every K > 1 entry: at most 8 factors
+    // The entries are the same for every thread of a group: they are read by scalar loads
+    // (uniform addresses, into SGPRs), four at a time with the next four in flight, and used
+    // as scalar operands -- no LDS staging, no barriers.  Renormalisation every 8 factors
+    // (entry j of the slice with j % 8 == 7) and after every K > 1 entry: at most 8 factors
     // between renormalisations, so the products stay within FP64's normal range whenever
     // |f_c + R_d|^2 is in [2^-120, 2^120] (scaling by powers of two is exact, so the period
     // does not change the product's rounding; an exact zero stays zero, giving -inf as log 0
     // does in the reference).
-    const int64_t n = e1 - e0;
+    const int64_t n = s1 - s0;
     const int64_t n4 = n & ~(int64_t)3;
     Entry nx[4];
     if (n4 > 0) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) nx[j] = ent[e0 + j];
+      for (int j = 0; j < 4; ++j) nx[j] = ent[s0 + j];
     }
     for (int64_t j0 = 0; j0 < n4; j0 += 4) {
       Entry cu[4];
@@ -395,7 +409,7 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       for (int j = 0; j < 4; ++j) cu[j] = nx[j];
       if (j0 + 4 < n4) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) nx[j] = ent[e0 + j0 + 4 + j];
+        for (int j = 0; j < 4; ++j) nx[j] = ent[s0 + j0 + 4 + j];
       }
       if (cu[0].k == 1. && cu[1].k == 1. && cu[2].k == 1. && cu[3].k == 1.) {
         // four single-photon entries, straight-line (the common case)
@@ -412,18 +426,39 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       }
     }
     for (int64_t j = n4; j < n; ++j) {
-      const Entry E = ent[e0 + j];
+      const Entry E = ent[s0 + j];
       const int K = (int)E.k;
       factor(E.rr, E.ri, K);
       if ((j & 7) == 7 || K > 1) renorm();
     }
-    // candidate values and the local first-occurrence argmax
+    // the groups' products: each normalised (at most 7 factors since the last
+    // renormalisation), then group 0 multiplies the mantissas in group order (each in
+    // [0.5, 1): no overflow or underflow) and adds the exponents
+    renorm();
+    if (grp > 0) {
+#pragma unroll
+      for (int u = 0; u < kPerThread; ++u) {
+        cpm[grp - 1][u][ct] = prod[u];
+        cpe[grp - 1][u][ct] = ex[u];
+      }
+    }
+    __syncthreads();
+    if (grp == 0) {
+#pragma unroll
+      for (int g = 0; g < kGroups - 1; ++g)
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) {
+          prod[u] = prod[u] * cpm[g][u][ct];
+          ex[u] += cpe[g][u][ct];
+        }
+    }
+    // candidate values and the local first-occurrence argmax (group 0)
     double bv = -INFINITY;
     int bi = 0x7fffffff;
 #pragma unroll
     for (int u = 0; u < kPerThread; ++u) {
       const int c = cidx[u];
-      if (c < 0) continue;
+      if (c < 0 || grp > 0) continue;
       const double mc = mean_w(cr[u], ci[kProbe ? u : 0], m1r, m1i, vr);
       const double slog = (log(prod[u]) + (double)ex[u] * M_LN2) / N;
       const double r = rescale < 0. ? kbar / mc : rescale;
@@ -444,14 +479,14 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       }
     }
     __syncthreads();
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == 0 && grp == 0) {
       wv[threadIdx.x >> 6] = bv;
       wi[threadIdx.x >> 6] = bi;
     }
     __syncthreads();
     double v0 = wv[0];
     int imax = wi[0];
-    for (int w = 1; w < kWaves; ++w)
+    for (int w = 1; w < kTeamWaves; ++w)
       if (cand_better(wv[w], wi[w], v0, imax)) {
         v0 = wv[w];
         imax = wi[w];
@@ -818,7 +853,7 @@ extern "C" int rsp_maxlp_logq_pixel(const rsp_maxlp* m, int64_t t, const double*
                                     int64_t ncand, double rescale, double* out,
                                     rsp_stream_t stream_) {
   if (!m || !cand || !out) return fail(kErrNull, "rsp_maxlp_logq_pixel: null pointer");
-  if (ncand < 1 || ncand > kThreads * kPerThread)
+  if (ncand < 1 || ncand > kTeam * kPerThread)
     return fail(kErrSize, "rsp_maxlp_logq_pixel: 1 <= ncand <= 1280");
   if (t < 0 || t >= m->npix) return fail(kErrSize, "rsp_maxlp_logq_pixel: pixel out of range");
   if (!std::binary_search(m->mask_pix.begin(), m->mask_pix.end(), (int32_t)t))
