@@ -390,12 +390,14 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
     };
     // The entries are the same for every thread of a group: they are read by scalar loads
     // (uniform addresses, into SGPRs), four at a time with the next four in flight, and used
-    // as scalar operands -- no LDS staging, no barriers.  Renormalisation every 8 factors
-    // (entry j of the slice with j % 8 == 7) and after every K > 1 entry: at most 8 factors
-    // between renormalisations, so the products stay within FP64's normal range whenever
-    // |f_c + R_d|^2 is in [2^-120, 2^120] (scaling by powers of two is exact, so the period
-    // does not change the product's rounding; an exact zero stays zero, giving -inf as log 0
-    // does in the reference).
+    // as scalar operands -- no LDS staging, no barriers.  Renormalisation before an entry
+    // whose K factors would bring more than 8 since the last one (nf counts them, a
+    // wave-uniform scalar): at most 8 factors between renormalisations, so the products stay
+    // within FP64's normal range whenever |f_c + R_d|^2 is in [2^-120, 2^120] (scaling by
+    // powers of two is exact, so where the renormalisations fall does not change the
+    // product's rounding; an exact zero stays zero, giving -inf as log 0 does in the
+    // reference).
+    int nf = 0;
     const int64_t n = s1 - s0;
     const int64_t n4 = n & ~(int64_t)3;
     Entry nx[4];
@@ -413,23 +415,35 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       }
       if (cu[0].k == 1. && cu[1].k == 1. && cu[2].k == 1. && cu[3].k == 1.) {
         // four single-photon entries, straight-line (the common case)
+        if (nf > 4) {
+          renorm();
+          nf = 0;
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) factor(cu[j].rr, cu[j].ri, 1);
-        if (j0 & 4) renorm();  // the block's last entry is j0 + 3, 7 mod 8
+        nf += 4;
       } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int K = (int)cu[j].k;
+          if (nf + K > 8) {
+            renorm();
+            nf = 0;
+          }
           factor(cu[j].rr, cu[j].ri, K);
-          if (((j0 + j) & 7) == 7 || K > 1) renorm();
+          nf += K;
         }
       }
     }
     for (int64_t j = n4; j < n; ++j) {
       const Entry E = ent[s0 + j];
       const int K = (int)E.k;
+      if (nf + K > 8) {
+        renorm();
+        nf = 0;
+      }
       factor(E.rr, E.ri, K);
-      if ((j & 7) == 7 || K > 1) renorm();
+      nf += K;
     }
     // the groups' products: each normalised (at most 7 factors since the last
     // renormalisation), then group 0 multiplies the mantissas in group order (each in
