@@ -12,12 +12,13 @@
 //     model pixels t whose detector index is p (checked with the exact forward formula
 //     in a 5x5 window), giving per pixel the frames with K_d(t) > 0.  Counting, scan and
 //     a radix sort by (t, d) make the lists deterministic.
-//   * mean_d w_dc = rescale*(|f_c|^2 + 2 Re(f_c conj(m1)) + m2) from the per-pixel moments
-//     m1 = mean_d R_d(t), m2 = mean_d |R_d(t)|^2 (exact in real arithmetic).
-//   * sum_d K_d log|f_c + R_d|^2 = log prod_d |f_c + R_d|^(2K_d): a running FP64 product
-//     with periodic frexp renormalisation, one log per candidate at the end.
-//   * 256 threads x 5 candidates per workgroup; first-occurrence argmax (NaN first, as
-//     NumPy/CuPy argmax) by wave shuffles and LDS.
+//   * mean_d w_dc = rescale*(|f_c + m1|^2 + v) from the per-pixel moments
+//     m1 = mean_d R_d(t), v = mean_d |R_d(t) - m1|^2 (two non-negative terms).
+//   * sum_d K_d log|f_c + R_d|^2 = log prod_d |f_c + R_d|^(2K_d): running FP64 products
+//     with frexp renormalisation, one log per candidate at the end.
+//   * 4 entry groups x 256 threads x 5 candidates per workgroup (each group a quarter of
+//     the pixel's entries, partial products combined in LDS); first-occurrence argmax
+//     (NaN first, as NumPy/CuPy argmax) by wave shuffles and LDS.
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
