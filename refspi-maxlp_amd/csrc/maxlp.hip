@@ -189,50 +189,45 @@ __global__ __launch_bounds__(256) void k_entries(Geo g, const uint64_t* __restri
   ent[i] = e;
 }
 
-// Per-pixel moments m1 = mean_d R_d(t) and v = mean_d |R_d(t) - m1|^2 (second pass over the
-// frames, so that mean_d |f + R_d|^2 = |f + m1|^2 + v is a sum of two non-negative terms
-// with no cancellation, whatever f).  Fixed reduction order (deterministic).
+// Per-pixel moments m1 = mean_d R_d(t) and v = mean_d |R_d(t) - m1|^2, so that
+// mean_d |f + R_d|^2 = |f + m1|^2 + v is a sum of two non-negative terms with no
+// cancellation, whatever f.  One pass over the frames with sums shifted by c = R_0(t) (the
+// first frame's term, the same for every thread): S1 = sum_d (R_d - c), S2 = sum_d
+// |R_d - c|^2, m1 = c + S1/N, v = S2/N - |S1/N|^2 (clamped at 0).  Relative to the two-pass
+// form the subtraction costs a factor 1 + |m1 - c|^2 / v in v's rounding error, a few ulps
+// for a frame of the set as the shift.  Fixed reduction order (deterministic).
 __global__ __launch_bounds__(256) void k_moments(Geo g, const int32_t* __restrict__ pixels,
                                                  int N, const double* __restrict__ sx,
                                                  const double* __restrict__ sy,
                                                  const double* __restrict__ dia,
                                                  double* __restrict__ mom) {
-  __shared__ double red[2][256];
+  __shared__ double red[3][256];
   const int t = pixels[blockIdx.x];
-  double a = 0., b = 0.;
+  double cr, ci;
+  fref(g, t, dia[0], sx[0], sy[0], cr, ci);
+  double a = 0., b = 0., q = 0.;
   for (int d = threadIdx.x; d < N; d += 256) {
     double rr, ri;
     fref(g, t, dia[d], sx[d], sy[d], rr, ri);
-    a += rr;
-    b += ri;
+    const double er = rr - cr, ei = ri - ci;
+    a += er;
+    b += ei;
+    q += er * er + ei * ei;
   }
   red[0][threadIdx.x] = a;
   red[1][threadIdx.x] = b;
+  red[2][threadIdx.x] = q;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (threadIdx.x < w)
-      for (int k = 0; k < 2; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-    __syncthreads();
-  }
-  const double m1r = red[0][0] / N, m1i = red[1][0] / N;
-  __syncthreads();
-  double q = 0.;
-  for (int d = threadIdx.x; d < N; d += 256) {
-    double rr, ri;
-    fref(g, t, dia[d], sx[d], sy[d], rr, ri);
-    const double er = rr - m1r, ei = ri - m1i;
-    q += er * er + ei * ei;
-  }
-  red[0][threadIdx.x] = q;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      for (int k = 0; k < 3; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    mom[4 * t + 0] = m1r;
-    mom[4 * t + 1] = m1i;
-    mom[4 * t + 2] = red[0][0] / N;
+    const double ar = red[0][0] / N, ai = red[1][0] / N;
+    mom[4 * t + 0] = cr + ar;
+    mom[4 * t + 1] = ci + ai;
+    mom[4 * t + 2] = fmax(red[2][0] / N - (ar * ar + ai * ai), 0.);
   }
 }
 
