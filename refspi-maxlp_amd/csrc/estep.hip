@@ -703,8 +703,8 @@ struct SegMap {
   int nseg[kMaxClass];
 };
 
-// Four lanes per frame: lane u of the group merges segments u, u+4, ...; the four partial
-// top-3 are then combined by shuffles.  The top two are selected by (value, lower index)
+// kMergeLanes lanes per frame: lane u of the group merges segments u, u + kMergeLanes, ...;
+// the partial top-3 are then combined by shuffles.  The top two are selected by (value, lower index)
 // and v3 is a maximum, so the result does not depend on the order of the merges.
 __device__ __forceinline__ void top3_merge(Top3& t, const Top3& p) {
   top3_insert(t, p.v1, p.h1);
@@ -712,13 +712,15 @@ __device__ __forceinline__ void top3_merge(Top3& t, const Top3& p) {
   if (p.v3 > t.v3) t.v3 = p.v3;
 }
 
+constexpr int kMergeLanes = 16;
+
 __global__ __launch_bounds__(256) void k_merge_top3(const Top3* __restrict__ partials, SegMap sm,
                                                     int Npad, Top3* __restrict__ running,
                                                     int first) {
   const int tid = blockIdx.x * 256 + threadIdx.x;
-  const int j = tid >> 2, u = tid & 3;
+  const int j = tid / kMergeLanes, u = tid % kMergeLanes;
   const bool live = j < Npad;   // no early return: the shuffles below need all lanes
-  const int gw = j >> 6;
+  const int gw = j >> 6;  // the frame's wave of sorted frames
   int nseg = 0;
   for (int c = sm.n - 1; c >= 0; --c)
     if (gw < sm.gw_end[c]) nseg = sm.nseg[c];
@@ -726,17 +728,17 @@ __global__ __launch_bounds__(256) void k_merge_top3(const Top3* __restrict__ par
   Top3 t = (first || u != 0 || !live) ? top3_empty() : running[j];
   // the partials of 8 of this lane's segments are loaded before any is merged
   constexpr int kU = 8;
-  for (int g0 = u; g0 < nseg; g0 += 4 * kU) {
+  for (int g0 = u; g0 < nseg; g0 += kMergeLanes * kU) {
     Top3 p[kU];
 #pragma unroll
     for (int v = 0; v < kU; ++v)
-      if (g0 + 4 * v < nseg) p[v] = partials[(int64_t)(g0 + 4 * v) * Npad + j];
+      if (g0 + kMergeLanes * v < nseg) p[v] = partials[(int64_t)(g0 + kMergeLanes * v) * Npad + j];
 #pragma unroll
     for (int v = 0; v < kU; ++v)
-      if (g0 + 4 * v < nseg) top3_merge(t, p[v]);
+      if (g0 + kMergeLanes * v < nseg) top3_merge(t, p[v]);
   }
 #pragma unroll
-  for (int off = 1; off < 4; off <<= 1) {
+  for (int off = 1; off < kMergeLanes; off <<= 1) {
     Top3 o;
     o.v1 = __shfl_xor(t.v1, off);
     o.v2 = __shfl_xor(t.v2, off);
@@ -2257,7 +2259,8 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
       }
       RSP_LAUNCH_CHECK("k_gather");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));
-      hipLaunchKernelGGL(k_merge_top3, dim3((unsigned)ceil_div(4 * e->Npad, 256)), dim3(256), 0, st,
+      hipLaunchKernelGGL(k_merge_top3, dim3((unsigned)ceil_div(kMergeLanes * e->Npad, 256)),
+                         dim3(256), 0, st,
                          e->d_partials, sm, (int)e->Npad, e->d_running, first);
       RSP_LAUNCH_CHECK("k_merge_top3");
       first = 0;
