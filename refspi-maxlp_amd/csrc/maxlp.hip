@@ -16,8 +16,8 @@
 //     m1 = mean_d R_d(t), v = mean_d |R_d(t) - m1|^2 (two non-negative terms).
 //   * sum_d K_d log|f_c + R_d|^2 = log prod_d |f_c + R_d|^(2K_d): running FP64 products
 //     with frexp renormalisation, one log per candidate at the end.
-//   * 4 entry groups x 256 threads x 5 candidates per workgroup (each group a quarter of
-//     the pixel's entries, partial products combined in LDS); first-occurrence argmax
+//   * 4 entry groups x 256 threads x 5 candidates per workgroup (each group every fourth
+//     of the pixel's entries, partial products combined in LDS); first-occurrence argmax
 //     (NaN first, as NumPy/CuPy argmax) by wave shuffles and LDS.
 #include <cstring>
 
@@ -41,7 +41,7 @@ constexpr int kCentre = 544;      // pattern index of 0 + 0j
 // on per-entry work
 constexpr int kTeam = 256;        // threads holding the candidates of one pixel
 constexpr int kPerThread = 5;     // candidates per thread (7 x 5 rows x 33 columns >= 33 x 33)
-// Entry groups: a pixel's workgroup is kGroups teams, each taking a contiguous quarter of the
+// Entry groups: a pixel's workgroup is kGroups teams, each taking every kGroups-th of the
 // pixel's entries for all candidates; their partial products are combined in LDS every
 // round.  Four waves per SIMD on one pixel keep the FP64 pipelines issuing where a single
 // wave per SIMD waits on its dependency chains: the heaviest pixels (tens of thousands of
@@ -94,6 +94,18 @@ __device__ __forceinline__ void fref(const Geo& g, int t, double dia, double sx,
   rr = f * cs;
   ri = f * sn;
 }
+
+// Entry keys: model pixel t (bits 47-63), photon count K of the piece (43-46), frame d
+// (18-42).  Sorted ascending, a pixel's entries run by count, then by frame: the pattern
+// search then meets long runs of K = 1 entries (its straight-line path) instead of a
+// K > 1 entry in a third of its blocks of four.
+constexpr int kKeyD = 18, kKeyK = 43, kKeyT = 47;
+__host__ __device__ inline uint64_t entry_key(int t, int K, int d) {
+  return ((uint64_t)t << kKeyT) | ((uint64_t)K << kKeyK) | ((uint64_t)d << kKeyD);
+}
+__device__ __forceinline__ int key_t(uint64_t k) { return (int)(k >> kKeyT); }
+__device__ __forceinline__ int key_k(uint64_t k) { return (int)((k >> kKeyK) & 15); }
+__device__ __forceinline__ int key_d(uint64_t k) { return (int)((k >> kKeyD) & ((1u << 25) - 1)); }
 
 // Pull-back of detector photons to model pixels.  Photon e belongs to frame fr[e] at
 // detector pixel pix[e] with count cnt[e].  mode 0: count matches; mode 1: write keys.
@@ -154,7 +166,7 @@ __global__ __launch_bounds__(256) void k_pull(Geo g, const int32_t* __restrict__
         int left = cnt[e];
         while (left > 0) {
           const int piece = left < kMaxPiece ? left : kMaxPiece;
-          if (mode) keys[w++] = ((uint64_t)t << 47) | ((uint64_t)d << 22) | (uint64_t)piece;
+          if (mode) keys[w++] = entry_key(t, piece, d);
           ++n;
           left -= piece;
         }
@@ -167,9 +179,9 @@ __global__ __launch_bounds__(256) void k_bounds(const uint64_t* __restrict__ key
                                                 int64_t* __restrict__ seg_end) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const int t = (int)(keys[i] >> 47);
-  if (i == 0 || (int)(keys[i - 1] >> 47) != t) seg_beg[t] = i;
-  if (i == n - 1 || (int)(keys[i + 1] >> 47) != t) seg_end[t] = i + 1;
+  const int t = key_t(keys[i]);
+  if (i == 0 || key_t(keys[i - 1]) != t) seg_beg[t] = i;
+  if (i == n - 1 || key_t(keys[i + 1]) != t) seg_end[t] = i + 1;
 }
 
 __global__ __launch_bounds__(256) void k_entries(Geo g, const uint64_t* __restrict__ keys,
@@ -180,9 +192,9 @@ __global__ __launch_bounds__(256) void k_entries(Geo g, const uint64_t* __restri
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const uint64_t k = keys[i];
-  const int t = (int)(k >> 47);
-  const int d = (int)((k >> 22) & ((1u << 25) - 1));
-  const int K = (int)(k & ((1u << 22) - 1));
+  const int t = key_t(k);
+  const int d = key_d(k);
+  const int K = key_k(k);
   Entry e;
   fref(g, t, dia[d], sx[d], sy[d], e.rr, e.ri);
   e.k = (double)K;
@@ -328,9 +340,10 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       cidx[u] = ix < 33 ? ix * 33 + ct % 33 : -1;
     }
   }
-  // this group's entries: a contiguous slice of the pixel's, a multiple of 8 long
-  const int64_t glen = (((e1 - e0) + kGroups - 1) / kGroups + 7) & ~(int64_t)7;
-  const int64_t s0 = min(e1, e0 + grp * glen), s1 = min(e1, s0 + glen);
+  // this group's entries: every kGroups-th of the pixel's, from its grp-th (the same mix
+  // of counts in every group)
+  const int64_t s0 = e0 + grp;
+  const int64_t gn = e1 > s0 ? (e1 - s0 + kGroups - 1) / kGroups : 0;
   for (int it = 0; it < (kProbe ? 1 : 10); ++it) {
     ++rounds;
     double cr[kPerThread], ci[kProbe ? kPerThread : 1], prod[kPerThread];
@@ -394,12 +407,12 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
     // product's rounding; an exact zero stays zero, giving -inf as log 0 does in the
     // reference).
     int nf = 0;
-    const int64_t n = s1 - s0;
+    const int64_t n = gn;
     const int64_t n4 = n & ~(int64_t)3;
     Entry nx[4];
     if (n4 > 0) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) nx[j] = ent[s0 + j];
+      for (int j = 0; j < 4; ++j) nx[j] = ent[s0 + (int64_t)kGroups * j];
     }
     for (int64_t j0 = 0; j0 < n4; j0 += 4) {
       Entry cu[4];
@@ -407,7 +420,7 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       for (int j = 0; j < 4; ++j) cu[j] = nx[j];
       if (j0 + 4 < n4) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) nx[j] = ent[s0 + j0 + 4 + j];
+        for (int j = 0; j < 4; ++j) nx[j] = ent[s0 + (int64_t)kGroups * (j0 + 4 + j)];
       }
       if (cu[0].k == 1. && cu[1].k == 1. && cu[2].k == 1. && cu[3].k == 1.) {
         // four single-photon entries, straight-line (the common case)
@@ -432,7 +445,7 @@ __global__ __launch_bounds__(kThreads) void k_pattern(
       }
     }
     for (int64_t j = n4; j < n; ++j) {
-      const Entry E = ent[s0 + j];
+      const Entry E = ent[s0 + (int64_t)kGroups * j];
       const int K = (int)E.k;
       if (nf + K > 8) {
         renorm();
@@ -532,8 +545,7 @@ __global__ __launch_bounds__(256) void k_photons_pixel(int t, const int64_t* __r
   const int64_t e0 = seg_beg[t], e1 = seg_end[t];
   for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
     const uint64_t k = keys[e];
-    const int d = (int)((k >> 22) & ((1u << 25) - 1));
-    atomicAdd(&k_out[d], (double)(k & ((1u << 22) - 1)));
+    atomicAdd(&k_out[key_d(k)], (double)key_k(k));
   }
 }
 
@@ -742,6 +754,8 @@ extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, co
                        m->d_nmatch, (const int64_t*)m->d_pos, m->d_keys);
     RSP_LAUNCH_CHECK("k_pull(fill)");
     size_t tb = 0;
+    // all 64 bits (bits 0-17 are zero; restricting the sort to bits 18-63 gave a wrong order
+    // of the entries here: the MaxLP oracle test failed)
     RSP_HIP(rocprim::radix_sort_keys(nullptr, tb, m->d_keys, m->d_keys2, (size_t)total, 0, 64, st));
     if ((rc = ensure_tmp(m, tb))) return rc;
     RSP_HIP(rocprim::radix_sort_keys(m->d_tmp, tb, m->d_keys, m->d_keys2, (size_t)total, 0, 64, st));
