@@ -120,7 +120,11 @@ int rsp_emc_set_frames(rsp_emc* plan, int64_t num_data, const int32_t* ones,
  *   9 = frames per lane of whole-frame waves (2, the default: each slot register holds
  *       the 16-bit LDS addresses of two frames, so a workgroup covers twice the frames per
  *       table load; 1 = one frame per lane; set before frames).  Only which lanes and
- *       registers hold a frame changes, never the result. */
+ *       registers hold a frame changes, never the result.
+ *  10 = side streams shared by a batch's slot-class gather launches (2, the default;
+ *       1 .. 4; 0 = all on the caller's stream): the classes run concurrently between the
+ *       batch's table build and its merge, on streams of default priority created by the
+ *       plan.  Never changes the result. */
 int rsp_emc_set_option(rsp_emc* plan, int key, int64_t value);
 
 /* Run the E-step for `model_c` (device, npix complex) with per-frame `scales` (device,

@@ -949,6 +949,9 @@ struct rsp_emc {
   int layout = 1;            // per-tile LDS layout of the table parts (0: identity)
   int bcast = 1;             // broadcast-aware slot schedule (0: distinct banks per slot)
   int fpl = 2;               // frames per lane of whole-frame waves (1 or 2)
+  int class_streams = 2;     // side streams the slot-class launches of a batch share (0: one)
+  hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[4] = {nullptr, nullptr, nullptr, nullptr};
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   struct Class {
     int cfg, n_tiles, gw0;
@@ -996,6 +999,14 @@ void free_all(rsp_emc* e) {
     if (p) hipFree(p);
   for (hipEvent_t ev : e->ev) hipEventDestroy(ev);
   e->ev.clear();
+  for (int k = 0; k < 4; ++k) {
+    if (e->side[k]) hipStreamDestroy(e->side[k]);
+    if (e->ev_join[k]) hipEventDestroy(e->ev_join[k]);
+    e->side[k] = nullptr;
+    e->ev_join[k] = nullptr;
+  }
+  if (e->ev_fork) hipEventDestroy(e->ev_fork);
+  e->ev_fork = nullptr;
 }
 
 // Bank-aware slot assignment for one group of lanes of one wave and one table part
@@ -1696,6 +1707,10 @@ extern "C" int rsp_emc_set_option(rsp_emc* e, int key, int64_t value) {
       if (value != 1 && value != 2) return fail(kErrSize, "frames per lane must be 1 or 2");
       e->fpl = (int)value;
       return kOk;
+    case 10:
+      if (value < 0 || value > 4) return fail(kErrSize, "class streams must be 0 .. 4");
+      e->class_streams = (int)value;
+      return kOk;
     default: return fail(kErrSize, "rsp_emc_set_option: unknown key");
   }
 }
@@ -2221,6 +2236,16 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
   RSP_HIP(hipMemsetAsync(e->d_absmax, 0, sizeof(unsigned) * e->n_scored, st));
   RSP_HIP(hipMemsetAsync(e->d_counters, 0, sizeof(unsigned) * 4, st));
   const size_t lds_bytes = 0;  // the gather's three part buffers are static LDS
+  // The slot classes of a batch read the same tables and write disjoint partials: with
+  // class_streams side streams (default priority) they run concurrently, so that the
+  // last, partly filled wave of one class's workgroups shares the CUs with the next
+  // class's instead of leaving them idle.  Fork after k_tables, join before the merge.
+  const int ncs = e->class_streams;
+  for (int k = 0; k < ncs; ++k)
+    if (!e->side[k]) RSP_HIP(hipStreamCreateWithFlags(&e->side[k], hipStreamNonBlocking));
+  for (int k = 0; k < ncs; ++k)
+    if (!e->ev_join[k]) RSP_HIP(hipEventCreateWithFlags(&e->ev_join[k], hipEventDisableTiming));
+  if (ncs > 0 && !e->ev_fork) RSP_HIP(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming));
   if (!e->exact_mode) {
     const int HC = 32;  // hypotheses per k_tables thread (same-process A/B: 8, 16, 32, 64,
                         // 128 -> 180, 168, 155-162, 163, 176 ms per iteration)
@@ -2246,18 +2271,27 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
       RSP_LAUNCH_CHECK("k_tables");
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 1], st));
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 2], st));
+      if (ncs > 0) {
+        RSP_HIP(hipEventRecord(e->ev_fork, st));
+        for (int k = 0; k < ncs; ++k) RSP_HIP(hipStreamWaitEvent(e->side[k], e->ev_fork, 0));
+      }
       for (int ci = 0; ci < sm.n; ++ci) {
         const auto& c = e->classes[ci];
         const GatherCfg& gc = kGatherCfgs[c.cfg];
         const int ns = nseg_of(c, nb);
         sm.gw_end[ci] = c.gw0 + c.n_tiles * gc.NW * gc.F;
         sm.nseg[ci] = ns;
+        hipStream_t cs = ncs > 0 ? e->side[ci % ncs] : st;
         hipLaunchKernelGGL(gc.fn, dim3((unsigned)(ns * c.n_tiles)), dim3(64 * gc.NW), lds_bytes,
-                           st, e->d_tables, e->TS, (int)hb, nb, ns, c.n_tiles, c.gw0,
+                           cs, e->d_tables, e->TS, (int)hb, nb, ns, c.n_tiles, c.gw0,
                            e->d_slots + c.slot_off, e->d_scale_sorted, e->d_init, e->R,
                            e->d_partials, (int)e->Npad, (const uint32_t*)e->d_lay);
       }
       RSP_LAUNCH_CHECK("k_gather");
+      for (int k = 0; k < ncs; ++k) {
+        RSP_HIP(hipEventRecord(e->ev_join[k], e->side[k]));
+        RSP_HIP(hipStreamWaitEvent(st, e->ev_join[k], 0));
+      }
       if (e->timing) RSP_HIP(hipEventRecord(e->ev[4 * ib + 3], st));
       hipLaunchKernelGGL(k_merge_top3, dim3((unsigned)ceil_div(kMergeLanes * e->Npad, 256)),
                          dim3(256), 0, st,
