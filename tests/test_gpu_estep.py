@@ -243,6 +243,42 @@ def test_estep_two_frames_per_lane_do_not_change_result(size, n):
     assert out[1][2]['lds_write_bytes_per_hyp'] < 0.8 * out[0][2]['lds_write_bytes_per_hyp']
 
 
+def test_estep_class_streams_do_not_change_result():
+    """Option 10 (a batch's slot-class gather launches on side streams, forked after the
+    table build and joined before the merge) changes only when the classes run: argmax
+    and FP64 log-likelihood are identical on the caller's stream alone and with one, two
+    and four side streams, over several batches (a batch of 40 hypotheses, so that the
+    tables buffer and the partials are reused while streams of the previous batch could
+    still be running if the join were missing)."""
+    import types
+    import torch
+    from refspi_amd import synth
+    from refspi_amd.dset import Dataset
+    from refspi_amd.emc import EMC
+    cfg = synth.SynthConfig(**{**synth.CONFIG_185.__dict__, 'num_data': 6000, 'seed': 29})
+    d = synth.make_frames_torch(cfg, device='cuda')
+    fr = types.SimpleNamespace(num_pix=d['num_pix'], bg=None, **{
+        k: d[k].cpu().numpy().astype('i4')
+        for k in ('ones', 'multi', 'place_ones', 'place_multi', 'count_multi')})
+    model = synth.model_scales(cfg)[0] * synth.object_ft(cfg)
+    grid = dict(shiftx=(-1., 1., 3), shifty=(-1., 1., 3), sphere_dia=(7., 9., 2))
+    emc = EMC(dataset=Dataset(frames=fr), size=cfg.size, num_rot=24, model=model, seed=1,
+              save=False, **grid)
+    emc.set_option(2, 40)
+    m = torch.as_tensor(emc.model, device='cuda')
+    out = []
+    for cs in (0, 1, 2, 4):
+        emc.set_option(10, cs)
+        emc._calculate_prob(m)
+        emc._get_rmax()
+        out.append((emc.rmax.copy(), emc.maxll.copy()))
+    for r, ll in out[1:]:
+        assert np.array_equal(out[0][0], r)
+        np.testing.assert_array_equal(out[0][1], ll)
+    with pytest.raises(Exception):
+        emc.set_option(10, 5)
+
+
 def test_estep_bank_models_full_parts():
     """The 64-bank and unscheduled slot layouts on 185^2 frames, whose first table part is
     full (12,512 entries): the zero pads of a full part stay inside its LDS buffer, and
