@@ -2203,15 +2203,22 @@ extern "C" int rsp_emc_estep(rsp_emc* e, const double* model_c, const double* sc
     e->batch = batch;
   }
   // segments per batch and slot class: at most 128 hypotheses per workgroup, and at least
-  // about 4 workgroups per CU in every class launch (a class with few tiles gets more
-  // hypothesis segments), so that every launch fills the 256 CUs.  Measured at 185^2 with
-  // 100k frames (6 classes of 4 to 40 tiles): 4 workgroups per CU alone 1.86 s per E-step,
-  // a fixed 64 / 96 / 128 / 160 / 192 / 256 / 512 segments 1.87 / 1.87 / 1.81 / 1.81 /
-  // 1.83 / 1.83 / 1.92 s, this rule 1.80 s.
+  // about 4 workgroups per CU -- in every class launch when the classes run one after
+  // another (a class with few tiles gets more hypothesis segments, so that every launch
+  // fills the 256 CUs), over the batch's classes together when they share side streams
+  // (class_streams >= 2: concurrent classes fill the CUs between them, and extra segments
+  // only add partials and workgroup starts).  Measured at 185^2 with 100k frames (8
+  // classes of 2 to 40 tiles), one stream: 4 workgroups per CU alone 1.86 s per E-step, a
+  // fixed 64 / 96 / 128 / 160 / 192 / 256 / 512 segments 1.87 / 1.87 / 1.81 / 1.81 / 1.83 /
+  // 1.83 / 1.92 s, the per-class rule 1.80 s; two class streams: the per-class rule 1.563
+  // s, a fixed 64 / 96 / 128 / 192 / 256 segments 1.596 / 1.537 / 1.527 / 1.572 / 1.589 s.
   if ((int)e->classes.size() > kMaxClass) return fail(kErrUnsupported, "too many slot classes");
+  int64_t tiles_all = 0;
+  for (const auto& c : e->classes) tiles_all += c.n_tiles;
   auto nseg_of = [&](const rsp_emc::Class& c, int64_t nb) {
+    const int64_t tiles = e->class_streams >= 2 ? std::max<int64_t>(1, tiles_all) : c.n_tiles;
     int64_t ns = e->opt_nseg > 0 ? e->opt_nseg
-                                 : std::max(ceil_div(4 * 256, c.n_tiles), ceil_div(nb, 128));
+                                 : std::max(ceil_div(4 * 256, tiles), ceil_div(nb, 128));
     ns = std::max<int64_t>(1, std::min<int64_t>(ns, nb));
     if (ns >= 8 && e->opt_nseg == 0) ns = ns / 8 * 8;
     return (int)ns;
