@@ -580,6 +580,9 @@ struct rsp_maxlp {
   unsigned long long* d_counters = nullptr;
   unsigned long long counters[2] = {0, 0};
   hipEvent_t ev[4] = {};
+  // k_moments on a side stream beside the photon lists' pull-back, sort and entries
+  hipStream_t side = nullptr;
+  hipEvent_t sev[2] = {};
   double pattern_ms = 0.;
 };
 
@@ -716,6 +719,22 @@ extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, co
   hipStream_t st = (hipStream_t)stream_;
   Geo g{m->size, m->npix, m->d_qx, m->d_qy, m->d_qrad};
   int rc;
+  const int nmask = (int)m->mask_pix.size(), nr = (int)m->rmask_pix.size();
+  // The per-pixel moments need only the frames' parameters: on a side stream they run beside
+  // the photon lists' pull-back, sort and entries, after everything queued on st so far (the
+  // previous run's pattern search read them; the parameters come from st), and the pattern
+  // search waits for them.
+  if (!m->side) RSP_HIP(hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking));
+  for (hipEvent_t& e : m->sev)
+    if (!e) RSP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  RSP_HIP(hipEventRecord(m->sev[0], st));
+  RSP_HIP(hipStreamWaitEvent(m->side, m->sev[0], 0));
+  if (nmask > 0) {
+    hipLaunchKernelGGL(k_moments, dim3(nmask), dim3(256), 0, m->side, g, m->d_mask_pix,
+                       (int)m->N, sx, sy, dia, m->d_mom);
+    RSP_LAUNCH_CHECK("k_moments");
+  }
+  RSP_HIP(hipEventRecord(m->sev[1], m->side));
   // model-space photon lists
   int64_t total = 0;
   if (m->nph > 0) {
@@ -766,12 +785,7 @@ extern "C" int rsp_maxlp_run(rsp_maxlp* m, double* model_c, const double* sx, co
                        m->d_keys2, total, sx, sy, dia, m->d_ent);
     RSP_LAUNCH_CHECK("k_entries");
   }
-  const int nmask = (int)m->mask_pix.size(), nr = (int)m->rmask_pix.size();
-  if (nmask > 0) {
-    hipLaunchKernelGGL(k_moments, dim3(nmask), dim3(256), 0, st, g, m->d_mask_pix, (int)m->N, sx,
-                       sy, dia, m->d_mom);
-    RSP_LAUNCH_CHECK("k_moments");
-  }
+  RSP_HIP(hipStreamWaitEvent(st, m->sev[1], 0));
   // the pixel lists of a phase, heaviest first: d_ord2 (sorted keys of k_order_keys)
   auto order = [&](const int32_t* pixels, int n) -> int {
     hipLaunchKernelGGL(k_order_keys, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, pixels,
@@ -915,6 +929,9 @@ extern "C" int rsp_maxlp_destroy(rsp_maxlp* m) {
                   m->d_ent, m->d_tmp};
   for (void* p : ptrs)
     if (p) hipFree(p);
+  if (m->side) hipStreamDestroy(m->side);
+  for (hipEvent_t e : m->sev)
+    if (e) hipEventDestroy(e);
   if (m->d_counters) hipFree(m->d_counters);
   for (hipEvent_t e : m->ev)
     if (e) hipEventDestroy(e);
