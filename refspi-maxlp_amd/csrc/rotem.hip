@@ -57,6 +57,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4v __attribute__((ext_vector_type(4)));
 typedef double f64x2v __attribute__((ext_vector_type(2)));
 constexpr double kPi = 3.14159265358979323846;
+constexpr double kLn2 = 0.69314718055994530942;
+// buffer resource word 3 for raw dword loads (32-bit data format)
+constexpr int kBufWord3 = 0x00020000;
 
 enum Mode { kFP32 = 0, kSplit = 1, kFP64 = 2 };
 constexpr int kGenSizeMax = 513;  // generated reference rows: |x - c| <= 256 exact in bf16
@@ -347,7 +350,9 @@ __global__ __launch_bounds__(256) void k_rot_lt(
   typedef typename std::conditional<sizeof(T) == 4, uint3, double2>::type T2;
   __shared__ int32_t s_j[4][64];
   __shared__ T s_k[4][64], s_rr[4][64], s_ri[4][64], s_rrl[4][64], s_ril[4][64];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index as a known-uniform value: the frame, its photon range and the photon
+  // loop's trip count are then scalar
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int64_t d = d0 + (int64_t)blockIdx.x * 4 + w;
   if (d >= d1) return;
   const int h = (int)blockIdx.y * 64 + lane;
@@ -355,7 +360,8 @@ __global__ __launch_bounds__(256) void k_rot_lt(
   const double scale = scales[d];
   const T sc = (T)scale;
   const double fsx = sx[d], fsy = sy[d], fd = dia[d];
-  double acc = 0.;
+  double acc = 0., acc2 = 0.;   // acc2: sum of K log2 W (FP32 form)
+  float kf = 0.f;               // FP32 form: sum of K over photons below the log floor
   const int64_t o0 = fo[d], o1 = fo[d + 1];
   for (int64_t base = o0; base < o1; base += 64) {
     const int64_t i = base + lane;
@@ -372,28 +378,44 @@ __global__ __launch_bounds__(256) void k_rot_lt(
     }
     __builtin_amdgcn_wave_barrier();
     const int n = (int)min((int64_t)64, o1 - base);
-    // eight photons' OT reads in flight per wave (the reads come from the MALL: latency, not
-    // bandwidth, bounds this loop)
+    if constexpr (sizeof(T) == 4) {
+      // The photon's pixel index comes by a scalar load (uniform address) and its OT row
+      // is a buffer resource formed on the scalar unit (the lane's 32-bit offset h * 12,
+      // below the row's Hs * 12 bytes, is the only vector part).  The log is the hardware base-2 log (inputs >= 1e-20 are normal numbers), its
+      // K-weighted sum taken in FP64 and multiplied by ln 2 once; photons below the floor
+      // are counted apart (kf) so that each adds exactly -1000 K.
+      const char* otb = (const char*)OT_;
+      const uint32_t lofs = (uint32_t)h * 12u;
 #pragma unroll 8
-    for (int q = 0; q < n; ++q) {
-      const T2 o = OT[(int64_t)s_j[w][q] * Hs];
-      const T kq = s_k[w][q], rr = s_rr[w][q], ri = s_ri[w][q];
-      T zr, zi;
-      if constexpr (sizeof(T) == 4) {
+      for (int q = 0; q < n; ++q) {
+        const char* row = otb + (int64_t)fj[base + q] * Hs * 12;
+        const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, Hs * 12, kBufWord3);
+        const auto ov = __builtin_amdgcn_raw_buffer_load_b96(rs, lofs, 0, 0);
+        const uint3 o = make_uint3(ov[0], ov[1], ov[2]);
+        const float kq = s_k[w][q], rr = s_rr[w][q], ri = s_ri[w][q];
         // the hi parts cancel exactly when O ~ -R (Sterbenz); the lo parts restore the
         // digits the FP32 roundings of O and R dropped
-        zr = (__uint_as_float(o.x) + rr) + (__uint_as_float(o.z << 16) + s_rrl[w][q]);
-        zi = (__uint_as_float(o.y) + ri) + (__uint_as_float(o.z & 0xffff0000u) + s_ril[w][q]);
-      } else {
-        zr = o.x + rr;
-        zi = o.y + ri;
+        const float zr = (__uint_as_float(o.x) + rr) + (__uint_as_float(o.z << 16) + s_rrl[w][q]);
+        const float zi =
+            (__uint_as_float(o.y) + ri) + (__uint_as_float(o.z & 0xffff0000u) + s_ril[w][q]);
+        const float wv = sc * (zr * zr + zi * zi);
+        const bool ok = wv >= 1.e-20f;
+        acc2 += (double)(ok ? kq * __builtin_amdgcn_logf(wv) : 0.f);
+        kf += ok ? 0.f : kq;
       }
-      const T wv = sc * (zr * zr + zi * zi);
-      acc += (double)(kq * log_floor<T>(wv));
+    } else {
+#pragma unroll 8
+      for (int q = 0; q < n; ++q) {
+        const T2 o = OT[(int64_t)s_j[w][q] * Hs];
+        const T kq = s_k[w][q];
+        const T zr = o.x + s_rr[w][q], zi = o.y + s_ri[w][q];
+        acc += kq * log_floor<T>(sc * (zr * zr + zi * zi));
+      }
     }
     __builtin_amdgcn_wave_barrier();
   }
   if (h >= H) return;
+  if constexpr (sizeof(T) == 4) acc = acc2 * kLn2 - 1000. * (double)kf;
   const T* xp = (const T*)X_ + (d - d0) * ldx + h;
   double x = 0.;
   for (int z = 0; z < nxs; ++z) x += (double)xp[z * xstride];
