@@ -380,8 +380,9 @@ __global__ __launch_bounds__(256) void k_rot_lt(
     const int n = (int)min((int64_t)64, o1 - base);
     if constexpr (sizeof(T) == 4) {
       // The photon's pixel index comes by a scalar load (uniform address) and its OT row
-      // is a buffer resource formed on the scalar unit (the lane's 32-bit offset h * 12,
-      // below the row's Hs * 12 bytes, is the only vector part).  The log is the hardware base-2 log (inputs >= 1e-20 are normal numbers), its
+      // is a buffer resource formed on the scalar unit (the lane's 32-bit offset h * 12 is
+      // the only vector part).  The resource covers the H hypotheses only: lanes of the
+      // last tile with h >= H (padding, not written) read 0 and fetch nothing.  The log is the hardware base-2 log (inputs >= 1e-20 are normal numbers), its
       // K-weighted sum taken in FP64 and multiplied by ln 2 once; photons below the floor
       // are counted apart (kf) so that each adds exactly -1000 K.
       const char* otb = (const char*)OT_;
@@ -389,7 +390,7 @@ __global__ __launch_bounds__(256) void k_rot_lt(
 #pragma unroll 8
       for (int q = 0; q < n; ++q) {
         const char* row = otb + (int64_t)fj[base + q] * Hs * 12;
-        const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, Hs * 12, kBufWord3);
+        const auto rs = __builtin_amdgcn_make_buffer_rsrc((void*)row, 0, H * 12, kBufWord3);
         const auto ov = __builtin_amdgcn_raw_buffer_load_b96(rs, lofs, 0, 0);
         const uint3 o = make_uint3(ov[0], ov[1], ov[2]);
         const float kq = s_k[w][q], rr = s_rr[w][q], ri = s_ri[w][q];
