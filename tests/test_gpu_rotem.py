@@ -142,3 +142,27 @@ def test_rot_generated_ref_rows_edge_diameters(small_frames, oracle_small, dias)
     ll, _, _ = plan.estep(o['model'], o['sx'], o['sy'], dia, o['sc'])
     err = np.abs(ll - LL) / np.abs(LL)
     assert err.max() < 1e-5, err.max()
+
+
+@pytest.mark.parametrize('mode', ['fp64', 'fp32', 'split'])
+def test_rot_log_floor_counts_each_photon(small_frames, oracle_small, mode):
+    """Frames scaled by 1e-30 put every photon's W = scale |O + R|^2 below the 1e-20 log
+    floor (kernels.cu:37-40: -1000 per count), so their LL is -1000 sum K up to a dense term
+    of ~1e-26, in the oracle and in every mode (the FP32 forms count floored photons apart
+    from their base-2 log sums); the other frames stay within the mode's bound."""
+    from oracle import rot_oracle as O
+    o = oracle_small
+    sc = o['sc'].copy()
+    sc[::5] = 1e-30
+    LL, _ = O.rot_loglik(o['model'], SMALL['size'], R, o['K'], o['sx'], o['sy'], o['dia'], sc,
+                         rel_scale=1.3)
+    floor = np.broadcast_to(-1000. * o['K'][::5].sum(1)[:, None], LL[::5].shape)
+    assert floor.min() < 0.
+    np.testing.assert_allclose(LL[::5], floor, rtol=1e-12, atol=1e-12)
+    plan = _plan(small_frames, mode)
+    ll, _, _ = plan.estep(o['model'], o['sx'], o['sy'], o['dia'], sc)
+    np.testing.assert_allclose(ll[::5], floor, rtol=1e-12, atol=1e-12)
+    keep = np.ones(len(sc), bool)
+    keep[::5] = False
+    err = np.abs(ll[keep] - LL[keep]) / np.abs(LL[keep])
+    assert err.max() < (1e-11 if mode == 'fp64' else 1e-5), err.max()
